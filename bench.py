@@ -1,0 +1,246 @@
+"""Benchmark: zone-updates/s of the Godunov block update on one MI355X.
+
+Workload (default): the locally isothermal binary minidisk -- [-8, 8]^2 as
+16 x 16 blocks of 256^2 zones (16.8 M zones), equal-mass circular binary
+(a = 1, GM = 1), softening 0.05, Mach 10, sink radius 0.05, Sigma = exp(0.01 z)
+seeded log-normal noise, Keplerian velocities, RK2.  The headline value is the
+mixed kernel (FP32 reconstruction and fluxes, FP64 conserved state); the FP64
+drop-in kernel is timed beside it, and the density L1 difference between the
+two after --l1-steps steps is reported.
+
+One "step" = one RK2 step (two fused stage launches) over the whole mesh,
+state resident in HBM.  A zone-update is one zone through one RK2 step
+(the reference's kzps convention, src/app_problem.cpp:84-88).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload minidisk|kh]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", choices=["minidisk", "kh"], default="minidisk")
+    ap.add_argument("--l1-steps", type=int, default=1000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    return ap.parse_args()
+
+
+# ---------------------------------------------------------------------------
+def make_solver(workload, precision):
+    from mara3p_amd import MinidiskData, problems
+    from mara3p_amd.solvers import Euler2DSolver, MinidiskSolver
+    if workload == "minidisk":
+        data = MinidiskData.defaults(block_size=256, depth=4, domain_radius=8.0, softening_length=0.05,
+                                     sink_radius=0.05, mach_number=10.0, omega_frame=1.0, mass_ratio=1.0,
+                                     eccentricity=0.0)
+        mesh, u0 = problems.minidisk(data, seed=1, noise=0.01)
+        s = MinidiskSolver(data, precision)
+        s.set_state(u0)
+        dt_of = lambda t: problems.minidisk_dt(data, t)
+        return s, u0, dt_of, 3
+    mesh, u0 = problems.kelvin_helmholtz(depth=4, block_size=256, seed=1)
+    s = Euler2DSolver(4, 256, gamma=5.0 / 3.0, plm_theta=1.5, precision=precision)
+    s.set_state(u0)
+    dt = 0.4 * s.dl / 2.0
+    return s, u0, (lambda t: dt), 5
+
+
+def timed_run(s, dt_of, steps, warmup):
+    """Time `steps` RK2 steps; per-launch durations from HIP events on the launch stream."""
+    import torch
+    for _ in range(warmup):
+        s.step(dt_of(getattr(s, "time", 1.0)))
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(2 * steps)]
+    orig_stage = s.stage
+    counter = [0]
+
+    def stage(*a, **k):
+        e0, e1 = ev[counter[0]]
+        e0.record(stream)
+        orig_stage(*a, **k)
+        e1.record(stream)
+        counter[0] += 1
+    s.stage = stage
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(dt_of(getattr(s, "time", 1.0)))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s.stage = orig_stage
+    s.check_errors()
+    durs = [a.elapsed_time(b) for a, b in ev]      # ms
+    return wall, float(np.mean(durs[0::2])), float(np.mean(durs[1::2]))
+
+
+# ---------------------------------------------------------------------------
+def cpu_baseline(workload, u0, dt):
+    """Reference CPU path on the host cores: the reference's own
+    minidisk::updated_conserved (oracle/_ref) when built, else the oracle,
+    one task per block per stage on a thread pool (the reference's scheduling,
+    src/parallel_computable.cpp:248-296).  Sample: one RK2 step of the mesh."""
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle
+    from oracle import OrcMinidisk, P, orc
+    from mara3p_amd import MinidiskData
+    from mara3p_amd.mesh import to_reference_layout
+    threads = min(os.cpu_count() or 1, 16)
+    if workload != "minidisk":
+        u = np.ascontiguousarray(to_reference_layout(u0))
+        nblk = u.shape[0]
+        t0 = time.perf_counter()
+        orc().orc_euler2d_mesh_rk2(P(u), 16, 256, dt, 1.0 / 4096, 1.5, 5.0 / 3.0, 0, 1, threads)
+        el = time.perf_counter() - t0
+        zones = nblk * 256 * 256
+        return {"value": zones / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+                "sample": "one RK2 step of the full 4096^2 mesh, oracle (C restatement, -O2), OpenMP over blocks"}
+    data = MinidiskData.defaults(block_size=256, depth=4, domain_radius=8.0, softening_length=0.05, sink_radius=0.05)
+    sd = OrcMinidisk.from_data(data)
+    sda = sd.as_array()
+    u = np.ascontiguousarray(to_reference_layout(u0))
+    nb, n = 16, 256
+    use_ref = oracle.ref_available()
+    kind = "reference" if use_ref else "port"
+
+    def stage(src, time_):
+        P_ = np.empty_like(src)
+        P_[..., 0] = src[..., 0]
+        P_[..., 1] = src[..., 1] / src[..., 0]
+        P_[..., 2] = src[..., 2] / src[..., 0]
+        P_ = np.ascontiguousarray(P_)
+        out = np.empty_like(src)
+
+        def task(b):
+            pe = np.empty((n + 4, n + 4, 3))
+            orc().orc_extend_block_2d(P(P_), nb, n, 3, b // nb, b % nb, P(pe))
+            o = np.empty((n, n, 3))
+            if use_ref:
+                oracle.ref().ref_minidisk_updated_conserved(P(np.ascontiguousarray(src[b])), P(pe), P(sda), 4,
+                                                            b // nb, b % nb, time_, dt, P(o))
+            else:
+                orc().orc_minidisk_updated_conserved(P(np.ascontiguousarray(src[b])), P(pe), ctypes.byref(sd), 4,
+                                                     b // nb, b % nb, time_, dt, P(o))
+            out[b] = o
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(task, range(nb * nb)))
+        return out
+
+    t0 = time.perf_counter()
+    u1 = stage(u, 0.0)
+    u2 = stage(u1, dt)
+    _ = u * 0.5 + u2 * 0.5
+    el = time.perf_counter() - t0
+    zones = nb * nb * n * n
+    what = ("reference minidisk::updated_conserved compiled from its sources (-O2)" if use_ref
+            else "oracle C restatement (-O2)")
+    return {"value": zones / el, "unit": "zone-updates/s", "cores": threads, "kind": kind,
+            "sample": "one RK2 step of the full 16x16x256^2 mesh: %s, one task per block per stage" % what}
+
+
+def traffic_from_profiles(workload, precision):
+    """HBM bytes per launch from committed rocprofv3 PMC summaries, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (workload, precision))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+
+    s, u0, dt_of, nc = make_solver(args.workload, "mixed")
+    zones = s.mesh.zones()
+    wall, d1, d2 = timed_run(s, dt_of, args.steps, args.warmup)
+    ms_per_step = 1e3 * wall / args.steps
+    value = zones * args.steps / wall
+    # algorithmic bytes: stage 1 reads U0 writes U1 (2 nc 8 B/zone), stage 2 reads U1, U0 writes U (3 nc 8 B/zone)
+    bytes_per_launch = 0.5 * (2 + 3) * nc * 8 * zones
+    kern_ms = 0.5 * (d1 + d2)
+    achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    result = {
+        "metric": "zone-updates/s, FP32-mixed vs FP64 GPU at matched L1 error; % of HBM peak",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-flux/f64-state",
+        "data": "synthetic",
+        "config": {},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles(args.workload, "mixed")},
+        "stage_ms": {"stage1": d1, "stage2": d2},
+    }
+    if args.workload == "minidisk":
+        result["config"] = {"workload": "minidisk iso2d+two_body, 16x16 blocks of 256^2 (16.8M zones), RK2, mixed",
+                            "zones": zones, "blocks": 256, "block_size": 256}
+    else:
+        result["config"] = {"workload": "euler2d Kelvin-Helmholtz 4096^2 periodic (16x16 blocks of 256^2), PLM+HLLE, RK2",
+                            "zones": zones, "blocks": 256, "block_size": 256}
+        result["dtype"] = "f32-flux/f64-state"
+
+    if not args.no_fp64:
+        s64, _, _, _ = make_solver(args.workload, "fp64")
+        k64 = max(10, args.steps // 4)
+        wall64, e1, e2 = timed_run(s64, dt_of, k64, max(2, args.warmup // 4))
+        v64 = zones * k64 / wall64
+        result["fp64"] = {"value": v64, "ms_per_step": 1e3 * wall64 / k64, "stage_ms": {"stage1": e1, "stage2": e2},
+                          "hbm_frac": bytes_per_launch / (0.5 * (e1 + e2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        result["mixed_over_fp64"] = value / v64
+        del s64
+
+    if args.l1_steps > 0 and args.workload == "minidisk":
+        a, ua, dtf, _ = make_solver("minidisk", "fp64")
+        b, _, _, _ = make_solver("minidisk", "mixed")
+        m0 = math.fsum(ua[:, 0].ravel())
+        for _ in range(args.l1_steps):
+            dt = dtf(a.time)
+            a.step(dt)
+            b.step(dt)
+        ra, rb = a.state(), b.state()
+        l1 = float(np.mean(np.abs(ra[:, 0] - rb[:, 0])))
+        result["l1_rho_mixed_vs_fp64"] = {"steps": args.l1_steps, "l1": l1,
+                                          "mass_fp64": math.fsum(ra[:, 0].ravel()) / m0,
+                                          "mass_mixed": math.fsum(rb[:, 0].ravel()) / m0}
+        del a, b
+
+    if not args.no_cpu_baseline:
+        try:
+            result["cpu_baseline"] = cpu_baseline(args.workload, u0, dt_of(1.0))
+        except Exception as e:  # the baseline is reported, never required
+            result["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+/*
+ * mara3p_amd: C ABI of the MI355X Godunov block update.
+ *
+ * All array arguments are DEVICE pointers (HIP, gfx950) unless marked
+ * "host".  Layout is structure-of-arrays per block, row-major with the last
+ * axis fastest, exactly the reference ndarray block with the per-zone tuple
+ * split into components in logical order:
+ *     state   [nblk][nc][n][n]          (2D)      [nblk][nc][n]   (1D)
+ *     pe      [nc][n+4][n+4]            primitive block with 2 guard zones
+ * Euler nc = 5 (D, S1, S2, S3, E / rho, v1, v2, v3, p); iso2d nc = 3.
+ *
+ * Every entry point returns an m3p status code.  Stream arguments are
+ * hipStream_t passed as void*.  Stage entry points are asynchronous on that
+ * stream; errors raised inside kernels (e.g. negative density) are latched in
+ * a per-device slot and reported by m3p_check_errors, which the per-block
+ * entry points call before returning (so they behave like the reference's
+ * throwing functions).
+ */
+#ifndef MARA3P_AMD_H
+#define MARA3P_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M3P_OK                     0
+#define M3P_ERR_NEGATIVE_DENSITY   1   /* euler::recover_primitive, src/physics_euler.hpp:158-161 */
+#define M3P_ERR_NEGATIVE_PRESSURE  2   /* mhd::recover_primitive, src/physics_mhd.hpp:296-300 */
+#define M3P_ERR_NO_CONVERGENCE     3   /* srhd::recover_primitive, src/physics_srhd.hpp:273-277 */
+#define M3P_ERR_BAD_SHAPE          4   /* nd::select / shape errors, src/core_ndarray.hpp:714-732 */
+#define M3P_ERR_HIP                5
+#define M3P_ERR_BAD_ARGUMENT       6
+
+#define M3P_FP64   0   /* drop-in: reproduces the reference FP64 arithmetic */
+#define M3P_MIXED  1   /* FP32 reconstruction and fluxes, FP64 conserved state */
+
+/* Library / error handling ------------------------------------------------ */
+int         m3p_version(void);
+const char* m3p_last_error(void);       /* text of the last failure on this thread */
+/* Synchronise `stream`, read and clear the device error latch.  *code is the
+ * kernel status (M3P_OK if none), block/zone locate the first offender. */
+int         m3p_check_errors(void* stream, int* code, int* block, int* zone);
+
+/* Per-block entry points (compatibility path) ---------------------------- */
+
+/* modules::euler2d::updated_conserved, src/module_euler.cpp:263-298
+ * (declared src/module_euler.hpp:182-190).  uc [5][n][n], pe [5][n+4][n+4]
+ * primitives, out [5][n][n].  hllc = 0 reproduces the reference (HLLE). */
+int m3p_euler2d_updated_conserved(const double* uc, const double* pe, double* out, int n,
+                                  double dt, double dl, double plm_theta, double gamma_law_index,
+                                  int hllc, int precision, void* stream);
+
+/* modules::euler1d::updated_conserved, src/module_euler.cpp:107-135, with the
+ * outflow boundary of src/mara.cpp:123-143 selected by the two flags (the
+ * reference applies only the lower one when a block is both first and last). */
+int m3p_euler1d_updated_conserved(const double* uc, const double* pe, double* out, int n,
+                                  double dt, double dl, double plm_theta, double gamma_law_index,
+                                  int outflow_lower, int outflow_upper, int precision, void* stream);
+
+/* Minidisk solver data, problems/minidisk.hpp:80-96. */
+typedef struct {
+    int    block_size;
+    int    depth;
+    double omega_frame;
+    double mach_number;
+    double kinematic_viscosity;   /* must be 0: viscous fluxes not built */
+    double domain_radius;
+    double softening_length;
+    double eccentricity;
+    double mass_ratio;
+    double sink_radius;
+    double sink_rate;
+    double buffer_rate;
+    double buffer_scale;
+    double cfl;
+} m3p_minidisk_data;
+
+/* minidisk::updated_conserved, problems/minidisk_scheme.cpp:294-355
+ * (declared problems/minidisk.hpp:164-170).  uc [3][n][n] conserved
+ * densities, pe [3][n+4][n+4] primitives, block (level, bi, bj). */
+int m3p_minidisk_updated_conserved(const double* uc, const double* pe, double* out,
+                                   int level, int bi, int bj, double time, double dt,
+                                   const m3p_minidisk_data* data /* host */, int precision, void* stream);
+
+/* locally_isothermal::updated_conserved, src/module_locally_isothermal.cpp:182-212
+ * with zero viscosity.  The reference's temperature mapping is a host
+ * callback; callers evaluate it at mesh_geometry.face_coordinates and pass
+ * cs2_x [(n+1)][n] and cs2_y [n][(n+1)] (device). */
+int m3p_locally_isothermal_updated_conserved(const double* uc, const double* pe, double* out, int n,
+                                   double dt, double dl, const double* cs2_x, const double* cs2_y,
+                                   int precision, void* stream);
+
+/* Whole-mesh stage entry points (throughput path) ------------------------- */
+/* One RK stage over every block of a mesh, guard fill fused: zones outside a
+ * block are read from its neighbours through nbr [nblk][4] (device) =
+ * {prev axis 0, next axis 0, prev axis 1, next axis 1} (the periodic
+ * next_on/prev_on of src/mesh_block_index.hpp:334-335 for uniform meshes).
+ * out = u0 * rk_b + L(u) * (1 - rk_b) when u0 != NULL, else L(u)
+ * (amr::weighted_sum_tree, src/mesh_amr.hpp:455-467; RK2 uses rk_b = 0.5).
+ * dt_dev, when non-NULL, overrides dt with a device-resident value. */
+int m3p_euler2d_stage(const double* u, const double* u0, double* out, const int* nbr,
+                      int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
+                      double plm_theta, double gamma_law_index, int hllc, int precision, void* stream);
+
+/* Minidisk stage: block coordinates bcoord [nblk][2] (device), all blocks at
+ * data->depth.  The binary's state at `time` is solved on the host
+ * (two_body::orbital_state + rotate, src/physics_two_body.hpp:109-157,417-421). */
+int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int* nbr, const int* bcoord,
+                       int nblk, double time, double dt, double rk_b,
+                       const m3p_minidisk_data* data /* host */, int precision, void* stream);
+
+/* 1D Euler stage over nblk blocks of n zones, out-flow at the two ends of the
+ * chain described by nbr [nblk][2] = {prev, next} (-1 = outflow boundary). */
+int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int* nbr,
+                      int nblk, int n, double dt, double dl, double rk_b,
+                      double plm_theta, double gamma_law_index, int precision, void* stream);
+
+/* Reductions --------------------------------------------------------------- */
+/* CFL time step for Euler states: dt = cfl * dl / max over zones of
+ * (max(|v1|, |v2|) + c_s), written to dt_out (device double).  The reduction
+ * is a wave64 shuffle max per workgroup and an atomic max across workgroups
+ * (positive doubles order like their bit patterns), so it is deterministic. */
+int m3p_euler_cfl_dt(const double* u, int nblk, long zones_per_block, double gamma_law_index,
+                     double cfl, double dl, double* dt_out, void* stream);
+
+/* FP64 totals of each conserved component over the mesh:
+ * totals[q] = sum over blocks and zones of u[b][q][...] (device double[nc]). */
+int m3p_conserved_totals(const double* u, int nblk, int nc, long zones_per_block, double* totals, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,65 @@
+"""Build recipe for the HIP library (gfx950) and the test oracle.
+
+    python -m mara3p_amd.build          # HIP library + oracle (+ reference harness when present)
+
+The HIP library is compiled in-tree so the .so travels with the repository
+snapshot.  Flags:
+  -ffp-contract=off        no implicit FMA: the FP64 kernels round exactly as
+                           the reference and shared face fluxes are bit-identical
+  -packed-fp32-ops off     no packed FP32 instructions
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "mara3p_amd")
+CSRC = os.path.join(PKG, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+REFERENCE = os.environ.get("MARA3P_REFERENCE", "/root/reference")
+
+HIP_SOURCES = ["capi.hip"]
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l and "argument unused" not in l)
+    if res.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), out))
+    if verbose and out.strip():
+        print(out)
+
+
+def build_hip(verbose=False):
+    target = os.path.join(PKG, "libmara3p_amd.so")
+    tmp = target + ".tmp"
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run([HIPCC] + HIP_FLAGS + srcs + ["-o", tmp], verbose)
+    os.replace(tmp, target)
+    return target
+
+
+def build_oracle(verbose=False):
+    oracle = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", oracle, os.path.join(oracle, "liboracle.so")], verbose)
+    if os.path.isdir(REFERENCE):
+        _run(["make", "-s", "-C", oracle, "REFERENCE=" + REFERENCE, "ref"], verbose)
+
+
+def build(verbose=False):
+    build_hip(verbose)
+    if shutil.which("make") and shutil.which("gcc"):
+        build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

@@ -1,0 +1,393 @@
+// C ABI of the MI355X Godunov block update (declarations: include/mara3p_amd.h).
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include "mara3p_amd.h"
+#include "stage1d.hpp"
+#include "stage2d.hpp"
+#include "two_body.hpp"
+
+using namespace m3p;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return M3P_OK;
+    return fail(M3P_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Per-device library state: the kernel error latch and reduction scratch.
+struct DeviceState {
+    int* err = nullptr;          // int[4]
+    unsigned long long* red = nullptr;  // reduction scratch, kRedWords words
+};
+
+constexpr int kMaxDevices = 64;
+constexpr int kRedWords = 4096;
+DeviceState g_dev[kMaxDevices];
+std::mutex g_mutex;
+
+int device_state(DeviceState** out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return fail(M3P_ERR_HIP, "m3p: no current HIP device");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState& s = g_dev[dev];
+    if (!s.err) {
+        if (int st = hip_check(hipMalloc(&s.err, 4 * sizeof(int)), "hipMalloc(error latch)")) return st;
+        if (int st = hip_check(hipMemset(s.err, 0, 4 * sizeof(int)), "hipMemset(error latch)")) return st;
+        if (int st = hip_check(hipMalloc(&s.red, kRedWords * sizeof(unsigned long long)), "hipMalloc(reduction)")) return st;
+    }
+    *out = &s;
+    return M3P_OK;
+}
+
+int launch_status(const char* what)
+{
+    return hip_check(hipGetLastError(), what);
+}
+
+// ---------------------------------------------------------------------------
+// 2D launch dispatch on block size: TJ = min(n, 256) threads per workgroup,
+// H = 32 rows per tile (16 for n = 16).
+template<class S, class Ld>
+int launch_stage2d(const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
+{
+    const int n = a.n;
+    auto go = [&] (auto kern, int TJ, int H) {
+        const int grid = a.nblk * (n / H) * (n / TJ);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(TJ), 0, s, prm, ld, a);
+        return launch_status("stage2d_kernel");
+    };
+    switch (n) {
+        case 16:  return go(stage2d_kernel<S, Ld, 16, 16>, 16, 16);
+        case 32:  return go(stage2d_kernel<S, Ld, 32, 32>, 32, 32);
+        case 64:  return go(stage2d_kernel<S, Ld, 32, 64>, 64, 32);
+        case 128: return go(stage2d_kernel<S, Ld, 32, 128>, 128, 32);
+        default:
+            if (n >= 256 && n % 256 == 0) return go(stage2d_kernel<S, Ld, 32, 256>, 256, 32);
+    }
+    return fail(M3P_ERR_BAD_SHAPE, "m3p: 2D block size must be 16, 32, 64, 128 or a multiple of 256 (got " + std::to_string(n) + ")");
+}
+
+template<typename R>
+int launch_stage1d(const EulerParams& prm, const Stage1DArgs& a, hipStream_t s)
+{
+    const int n = a.n;
+    auto go = [&] (auto kern, int TJ) {
+        hipLaunchKernelGGL(kern, dim3(a.nblk * (n / TJ)), dim3(TJ), 0, s, prm, a);
+        return launch_status("stage1d_kernel");
+    };
+    switch (n) {
+        case 16:  return go(stage1d_kernel<R, 16>, 16);
+        case 32:  return go(stage1d_kernel<R, 32>, 32);
+        case 64:  return go(stage1d_kernel<R, 64>, 64);
+        case 128: return go(stage1d_kernel<R, 128>, 128);
+        default:
+            if (n >= 256 && n % 256 == 0) return go(stage1d_kernel<R, 256>, 256);
+    }
+    return fail(M3P_ERR_BAD_SHAPE, "m3p: 1D block size must be 16, 32, 64, 128 or a multiple of 256 (got " + std::to_string(n) + ")");
+}
+
+MinidiskParams minidisk_params(const m3p_minidisk_data* d, double time)
+{
+    MinidiskParams p;
+    p.R = d->domain_radius;
+    p.rs = d->softening_length;
+    p.s2 = std::pow(d->softening_length, 2.0);
+    p.mach = d->mach_number;
+    p.omega = d->omega_frame;
+    p.sink_rate = d->sink_rate;
+    p.s6 = std::pow(d->sink_radius, 6.0);
+    p.buffer_rate = d->buffer_rate;
+    p.buffer_scale = d->buffer_scale;
+    double st[10];
+    two_body_state(1.0, 1.0, d->mass_ratio, d->eccentricity, time, d->omega_frame, st);
+    p.m1 = st[0]; p.x1 = st[1]; p.y1 = st[2];
+    p.m2 = st[5]; p.x2 = st[6]; p.y2 = st[7];
+    p.level = d->depth;
+    return p;
+}
+
+int check_minidisk_data(const m3p_minidisk_data* d)
+{
+    if (!d) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: minidisk data is NULL");
+    if (d->kinematic_viscosity != 0.0)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p: minidisk viscous fluxes are not implemented (nu must be 0)");
+    if (d->depth < 0 || d->depth > 20) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad minidisk depth");
+    return M3P_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+__global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(256)
+euler_max_speed_kernel(const double* u, int nblk, long zpb, EulerParams prm, unsigned long long* slot)
+{
+    __shared__ double part[4];
+    double m = 0.0;
+    const long total = (long)nblk * zpb;
+    for (long z = (long)blockIdx.x * blockDim.x + threadIdx.x; z < total; z += (long)gridDim.x * blockDim.x) {
+        const long b = z / zpb, i = z % zpb;
+        const double* base = u + b * 5 * zpb + i;
+        double uu[5] = { base[0], base[zpb], base[2 * zpb], base[3 * zpb], base[4 * zpb] };
+        m = fmax(m, Euler<double>::max_speed(uu, prm));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = part[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, part[w]);
+        atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+__global__ void cfl_finish_kernel(const unsigned long long* slot, double cfl, double dl, double* dt_out)
+{
+    double vmax = __longlong_as_double((long long)*slot);
+    *dt_out = cfl * dl / vmax;
+}
+
+// Two-level deterministic sum: per-workgroup partials (fixed order), then
+// one workgroup adds the partials in index order.
+__global__ void __launch_bounds__(256)
+totals_partial_kernel(const double* u, int nblk, int nc, long zpb, double* partial)
+{
+    // grid: (nparts, nc)
+    __shared__ double part[4];
+    const int q = blockIdx.y;
+    const long total = (long)nblk * zpb;
+    double s = 0.0;
+    for (long z = (long)blockIdx.x * blockDim.x + threadIdx.x; z < total; z += (long)gridDim.x * blockDim.x) {
+        const long b = z / zpb, i = z % zpb;
+        s += u[(b * nc + q) * zpb + i];
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += part[w];
+        partial[(size_t)q * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+__global__ void __launch_bounds__(64)
+totals_final_kernel(const double* partial, int nparts, double* totals)
+{
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 64) s += partial[(size_t)q * nparts + i];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) totals[q] = s;
+}
+
+} // namespace
+
+// ===========================================================================
+extern "C" {
+
+int m3p_version(void) { return 1; }
+
+const char* m3p_last_error(void) { return g_last_error.c_str(); }
+
+int m3p_check_errors(void* stream, int* code, int* block, int* zone)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    hipStream_t s = (hipStream_t)stream;
+    int h[4] = { 0, 0, 0, 0 };
+    if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
+    if (int st = hip_check(hipMemcpy(h, ds->err, sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy(error latch)")) return st;
+    if (h[0] != 0)
+        if (int st = hip_check(hipMemset(ds->err, 0, sizeof(h)), "hipMemset(error latch)")) return st;
+    if (code) *code = h[0];
+    if (block) *block = h[1];
+    if (zone) *zone = h[2];
+    return M3P_OK;
+}
+
+static int finish_blocking(void* stream)
+{
+    int code = 0, block = 0, zone = 0;
+    if (int st = m3p_check_errors(stream, &code, &block, &zone)) return st;
+    if (code == M3P_ERR_NEGATIVE_DENSITY)
+        return fail(code, "euler::recover_primitive (negative density)");
+    if (code) return fail(code, "m3p: kernel error " + std::to_string(code));
+    return M3P_OK;
+}
+
+int m3p_euler2d_updated_conserved(const double* uc, const double* pe, double* out, int n,
+                                  double dt, double dl, double plm_theta, double gamma_law_index,
+                                  int hllc, int precision, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!uc || !pe || !out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: NULL array");
+    EulerParams prm{ gamma_law_index, plm_theta, hllc };
+    StageArgs a{};
+    a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = ds->err;
+    hipStream_t s = (hipStream_t)stream;
+    int st;
+    if (precision == M3P_FP64)
+        st = launch_stage2d<EulerScheme<double>>(prm, ExtendedLoader<EulerScheme<double>>{uc, pe, n}, a, s);
+    else
+        st = launch_stage2d<EulerScheme<float>>(prm, ExtendedLoader<EulerScheme<float>>{uc, pe, n}, a, s);
+    if (st) return st;
+    return finish_blocking(stream);
+}
+
+int m3p_euler1d_updated_conserved(const double* uc, const double* pe, double* out, int n,
+                                  double dt, double dl, double plm_theta, double gamma_law_index,
+                                  int outflow_lower, int outflow_upper, int precision, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!uc || !pe || !out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: NULL array");
+    EulerParams prm{ gamma_law_index, plm_theta, 0 };
+    Stage1DArgs a{};
+    a.u = uc; a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = ds->err;
+    a.pe = pe; a.outflow_lower = outflow_lower; a.outflow_upper = outflow_upper;
+    hipStream_t s = (hipStream_t)stream;
+    int st = precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s) : launch_stage1d<float>(prm, a, s);
+    if (st) return st;
+    return finish_blocking(stream);
+}
+
+int m3p_minidisk_updated_conserved(const double* uc, const double* pe, double* out,
+                                   int level, int bi, int bj, double time, double dt,
+                                   const m3p_minidisk_data* data, int precision, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (int st = check_minidisk_data(data)) return st;
+    if (!uc || !pe || !out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: NULL array");
+    const int n = data->block_size;
+    MinidiskParams prm = minidisk_params(data, time);
+    prm.level = level;
+    StageArgs a{};
+    a.out = out; a.dt = dt; a.n = n; a.nblk = 1; a.err = ds->err;
+    a.bi0 = bi; a.bj0 = bj;
+    // cell_size, problems/minidisk_scheme.cpp:78-81
+    a.dl = 2.0 * data->domain_radius / double(data->block_size) / double(1 << level);
+    hipStream_t s = (hipStream_t)stream;
+    int st;
+    if (precision == M3P_FP64)
+        st = launch_stage2d<MinidiskScheme<double>>(prm, ExtendedLoader<MinidiskScheme<double>>{uc, pe, n}, a, s);
+    else
+        st = launch_stage2d<MinidiskScheme<float>>(prm, ExtendedLoader<MinidiskScheme<float>>{uc, pe, n}, a, s);
+    if (st) return st;
+    return finish_blocking(stream);
+}
+
+int m3p_euler2d_stage(const double* u, const double* u0, double* out, const int* nbr,
+                      int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
+                      double plm_theta, double gamma_law_index, int hllc, int precision, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !out || !nbr || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad euler2d stage arguments");
+    if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
+    EulerParams prm{ gamma_law_index, plm_theta, hllc };
+    StageArgs a{};
+    a.u0 = u0; a.out = out; a.dt_dev = dt_dev; a.dt = dt; a.dl = dl; a.rk_b = rk_b;
+    a.n = n; a.nblk = nblk; a.err = ds->err;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == M3P_FP64)
+        return launch_stage2d<EulerScheme<double>>(prm, MeshLoader<EulerScheme<double>>{u, nbr, n}, a, s);
+    return launch_stage2d<EulerScheme<float>>(prm, MeshLoader<EulerScheme<float>>{u, nbr, n}, a, s);
+}
+
+int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int* nbr, const int* bcoord,
+                       int nblk, double time, double dt, double rk_b,
+                       const m3p_minidisk_data* data, int precision, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (int st = check_minidisk_data(data)) return st;
+    if (!u || !out || !nbr || !bcoord || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad minidisk stage arguments");
+    if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
+    const int n = data->block_size;
+    MinidiskParams prm = minidisk_params(data, time);
+    StageArgs a{};
+    a.u0 = u0; a.out = out; a.dt = dt; a.rk_b = rk_b; a.bcoord = bcoord;
+    a.n = n; a.nblk = nblk; a.err = ds->err;
+    a.dl = 2.0 * data->domain_radius / double(data->block_size) / double(1 << data->depth);
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == M3P_FP64)
+        return launch_stage2d<MinidiskScheme<double>>(prm, MeshLoader<MinidiskScheme<double>>{u, nbr, n}, a, s);
+    return launch_stage2d<MinidiskScheme<float>>(prm, MeshLoader<MinidiskScheme<float>>{u, nbr, n}, a, s);
+}
+
+int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int* nbr,
+                      int nblk, int n, double dt, double dl, double rk_b,
+                      double plm_theta, double gamma_law_index, int precision, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !out || !nbr || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad euler1d stage arguments");
+    if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
+    EulerParams prm{ gamma_law_index, plm_theta, 0 };
+    Stage1DArgs a{};
+    a.u = u; a.u0 = u0; a.out = out; a.nbr = nbr; a.dt = dt; a.dl = dl; a.rk_b = rk_b;
+    a.n = n; a.nblk = nblk; a.err = ds->err;
+    hipStream_t s = (hipStream_t)stream;
+    return precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s) : launch_stage1d<float>(prm, a, s);
+}
+
+int m3p_euler_cfl_dt(const double* u, int nblk, long zones_per_block, double gamma_law_index,
+                     double cfl, double dl, double* dt_out, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !dt_out || nblk <= 0 || zones_per_block <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad cfl arguments");
+    hipStream_t s = (hipStream_t)stream;
+    EulerParams prm{ gamma_law_index, 0.0, 0 };
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, ds->red, 1, 0ull);
+    const long total = (long)nblk * zones_per_block;
+    int grid = (int)std::min<long>((total + 255) / 256, 2048);
+    hipLaunchKernelGGL(euler_max_speed_kernel, dim3(grid), dim3(256), 0, s, u, nblk, zones_per_block, prm, ds->red);
+    hipLaunchKernelGGL(cfl_finish_kernel, dim3(1), dim3(1), 0, s, ds->red, cfl, dl, dt_out);
+    return launch_status("m3p_euler_cfl_dt");
+}
+
+int m3p_conserved_totals(const double* u, int nblk, int nc, long zones_per_block, double* totals, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !totals || nblk <= 0 || nc <= 0 || nc > 8 || zones_per_block <= 0)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad totals arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int nparts = 256;     // nc <= 8 components x 256 partials fit in the scratch
+    double* partial = reinterpret_cast<double*>(ds->red);
+    hipLaunchKernelGGL(totals_partial_kernel, dim3(nparts, nc), dim3(256), 0, s, u, nblk, nc, zones_per_block, partial);
+    hipLaunchKernelGGL(totals_final_kernel, dim3(nc), dim3(64), 0, s, partial, nparts, totals);
+    return launch_status("m3p_conserved_totals");
+}
+
+int m3p_locally_isothermal_updated_conserved(const double*, const double*, double*, int,
+                                   double, double, const double*, const double*, int, void*)
+{
+    return fail(M3P_ERR_BAD_ARGUMENT, "m3p_locally_isothermal_updated_conserved: not built yet");
+}
+
+} // extern "C"

@@ -1,0 +1,91 @@
+// Device helpers shared by every kernel.  CDNA4 (gfx950), wave64.
+//
+// All HIP code is compiled with -ffp-contract=off: no FMA is formed unless it
+// is written explicitly, so the FP64 kernels round exactly as the reference's
+// x86-64 build and every face flux is bit-identical wherever it is computed
+// (which keeps fluxes shared by two tiles conservative).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+namespace m3p {
+
+constexpr int kWave = 64;
+
+// std::max(a, b) == (a < b) ? b : a ; std::min(a, b) == (b < a) ? b : a
+template<typename R> __device__ __forceinline__ R smax(R a, R b) { return (a < b) ? b : a; }
+template<typename R> __device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
+
+template<typename R> struct is_fp64 : std::is_same<R, double> {};
+
+// Generalised-minmod PLM slope, src/scheme_plm_gradient.hpp:59-67 with
+// coordinates (0, 1, 2).  The divisions by 1.0 and 2.0 there are exact, so
+// they are written as the identical-valued forms below.
+template<typename R>
+__device__ __forceinline__ R plm(R yl, R y0, R yr, R theta)
+{
+    R a = (y0 - yl) * theta;
+    R b = (yr - yl) * R(0.5);
+    R c = (yr - y0) * theta;
+    if constexpr (is_fp64<R>::value) {
+        R sa = copysign(R(1), a), sb = copysign(R(1), b), sc = copysign(R(1), c);
+        R m = smin(smin(fabs(a), fabs(b)), fabs(c));
+        return R(0.25) * fabs(sa + sb) * (sa + sc) * m;
+    } else {
+        // Same value: non-zero only when a, b, c share a sign; then sign(a) * min|.|.
+        R m = fminf(fminf(fabsf(a), fabsf(b)), fabsf(c));
+        bool same = (signbit(a) == signbit(b)) && (signbit(a) == signbit(c));
+        return same ? copysignf(m, a) : R(0);
+    }
+}
+
+// Division / square root: IEEE for FP64 (parity), fast for FP32 (mixed).
+template<typename R> __device__ __forceinline__ R fdiv(R a, R b)
+{
+    if constexpr (is_fp64<R>::value) return a / b;
+    else return a * __builtin_amdgcn_rcpf(b);
+}
+template<typename R> __device__ __forceinline__ R frcp(R b)
+{
+    if constexpr (is_fp64<R>::value) return R(1) / b;
+    else return __builtin_amdgcn_rcpf(b);
+}
+template<typename R> __device__ __forceinline__ R fsqrt(R a)
+{
+    if constexpr (is_fp64<R>::value) return sqrt(a);
+    else return __builtin_amdgcn_sqrtf(a);
+}
+
+// Error slot: err[0] = status code (first writer wins), err[1] = block, err[2] = zone.
+__device__ __forceinline__ void report_error(int* err, int code, int block, int zone)
+{
+    if (err && atomicCAS(&err[0], 0, code) == 0) {
+        err[1] = block;
+        err[2] = zone;
+    }
+}
+
+// Wave64 reductions through DPP/permute shuffles.
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// XCD-aware workgroup remap.  Workgroups are dispatched round-robin over the
+// 8 XCDs (each with its own L2); give each XCD a contiguous range of logical
+// tiles so that tiles sharing halo rows share an L2.
+__device__ __forceinline__ int xcd_remap(int w, int total)
+{
+    constexpr int nxcd = 8;
+    if (total % nxcd != 0) return w;
+    return (w % nxcd) * (total / nxcd) + (w / nxcd);
+}
+
+} // namespace m3p

@@ -1,0 +1,220 @@
+// Block-update schemes for the 2D stage kernel: what happens at a face
+// (Riemann flux) and at a zone (conserved update + source terms).
+//
+//   EulerScheme     <- modules::euler2d::updated_conserved   (src/module_euler.cpp:263-298)
+//   MinidiskScheme  <- minidisk::updated_conserved           (problems/minidisk_scheme.cpp:294-355)
+//   IsoTableScheme  <- locally_isothermal::updated_conserved (src/module_locally_isothermal.cpp:182-212),
+//                      face sound speeds supplied as arrays (the reference's
+//                      temperature mapping is a host std::function).
+#pragma once
+#include "physics.hpp"
+
+namespace m3p {
+
+// Per-tile context handed to the scheme hooks.
+struct TileCtx {
+    int b;      // storage block index
+    int bi, bj; // block coordinates on its level
+    int k0, c0; // first row / column of the tile inside the block
+    int n;      // block size
+};
+
+// ---------------------------------------------------------------------------
+template<typename R>
+struct EulerScheme {
+    static constexpr int NC = 5;
+    using Real = R;
+    using Params = EulerParams;
+    template<int H, int TJ> struct TileData {};
+
+    __device__ static __forceinline__ double theta(const Params& prm) { return prm.theta; }
+    __device__ static __forceinline__ bool recover(const double* u, const Params& prm, R* p)
+    {
+        return Euler<R>::recover(u, prm, p);
+    }
+    template<int H, int TJ>
+    __device__ static __forceinline__ void setup(TileData<H, TJ>&, const Params&, const TileCtx&) {}
+
+    template<int AXIS, int H, int TJ>
+    __device__ static __forceinline__ void face(const R* pl, const R* pr, const Params& prm,
+                                                const TileData<H, TJ>&, const TileCtx&, int, int, R* f)
+    {
+        Euler<R>::template riemann<AXIS>(pl, pr, prm, f);
+    }
+
+    // U - ((dFx + dFy) * dt) / dl   (src/module_euler.cpp:297)
+    template<int H, int TJ>
+    __device__ static __forceinline__ void update(const Params&, const TileData<H, TJ>&, const TileCtx&,
+                                                  int, int, const double* u, const R*,
+                                                  const double* dfx, const double* dfy,
+                                                  double dt, double dl, double* out)
+    {
+        if constexpr (is_fp64<R>::value) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) out[q] = u[q] - (dfx[q] + dfy[q]) * dt / dl;
+        } else {
+            double dtdl = dt / dl;
+#pragma unroll
+            for (int q = 0; q < NC; ++q) out[q] = u[q] - (dfx[q] + dfy[q]) * dtdl;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+struct MinidiskParams {
+    double R;            // domain radius
+    double rs;           // softening length
+    double s2;           // pow(rs, 2) evaluated on the host (std::pow)
+    double mach;
+    double omega;        // frame rotation rate
+    double sink_rate;
+    double s6;           // pow(sink_radius, 6) evaluated on the host
+    double buffer_rate;
+    double buffer_scale;
+    double m1, x1, y1;   // binary components at this stage's time
+    double m2, x2, y2;
+    int level;
+};
+
+template<typename R>
+struct MinidiskScheme {
+    static constexpr int NC = 3;
+    using Real = R;
+    using Params = MinidiskParams;
+    template<int H, int TJ> struct TileData { double xv[H + 1]; double yv[TJ + 1]; };
+
+    // theta = 2 is hard-coded by the reference (problems/minidisk_scheme.cpp:313-314)
+    __device__ static __forceinline__ double theta(const Params&) { return 2.0; }
+    __device__ static __forceinline__ bool recover(const double* u, const Params&, R* p)
+    {
+        Iso2d<R>::recover(u, p);
+        return true;
+    }
+
+    // vertex_positions (problems/minidisk_scheme.cpp:101-116) + nd::linspace
+    __device__ static __forceinline__ double vertex(double Rd, int n, int level, int b, int i)
+    {
+        double dlev = double(1 << level);
+        double x0 = Rd * (-1 + double(2 * (unsigned long)b) / dlev);
+        double x1 = Rd * (-1 + double(2 * (unsigned long)(b + 1)) / dlev);
+        return x0 + (x1 - x0) * double(i) / double(n);
+    }
+
+    template<int H, int TJ>
+    __device__ static __forceinline__ void setup(TileData<H, TJ>& td, const Params& prm, const TileCtx& tc)
+    {
+        for (int t = threadIdx.x; t <= H; t += blockDim.x) td.xv[t] = vertex(prm.R, tc.n, prm.level, tc.bi, tc.k0 + t);
+        for (int t = threadIdx.x; t <= TJ; t += blockDim.x) td.yv[t] = vertex(prm.R, tc.n, prm.level, tc.bj, tc.c0 + t);
+    }
+
+    // sound_speed_squared :127-132 at face_coordinates :134-151
+    template<int AXIS, int H, int TJ>
+    __device__ static __forceinline__ void face(const R* pl, const R* pr, const Params& prm,
+                                                const TileData<H, TJ>& td, const TileCtx& tc, int fi, int fj, R* f)
+    {
+        double x, y;
+        if constexpr (AXIS == 1) {
+            x = td.xv[fi - tc.k0];
+            y = 0.5 * (td.yv[fj - tc.c0] + td.yv[fj - tc.c0 + 1]);
+        } else {
+            x = 0.5 * (td.xv[fi - tc.k0] + td.xv[fi - tc.k0 + 1]);
+            y = td.yv[fj - tc.c0];
+        }
+        R cs2;
+        if constexpr (is_fp64<R>::value) {
+            double r2 = x * x + y * y;
+            double ph = -1.0 * 1.0 / sqrt(r2 + prm.s2);
+            cs2 = -ph / prm.mach / prm.mach;
+        } else {
+            float xf = float(x), yf = float(y);
+            float q = fmaf(xf, xf, fmaf(yf, yf, float(prm.s2)));
+            cs2 = __builtin_amdgcn_rsqf(q) * float(1.0 / (prm.mach * prm.mach));
+        }
+        Iso2d<R>::template hlle<AXIS>(pl, pr, cs2, f);
+    }
+
+    // The source-term sum and update of problems/minidisk_scheme.cpp:329-352.
+    template<int H, int TJ>
+    __device__ static __forceinline__ void update(const Params& prm, const TileData<H, TJ>& td, const TileCtx& tc,
+                                                  int k, int c, const double* u, const R* pc,
+                                                  const double* dfx, const double* dfy,
+                                                  double dt, double dl, double* out)
+    {
+        double x = 0.5 * (td.xv[k - tc.k0] + td.xv[k - tc.k0 + 1]);
+        double y = 0.5 * (td.yv[c - tc.c0] + td.yv[c - tc.c0 + 1]);
+        if constexpr (is_fp64<R>::value) {
+            // two_body::gravitational_acceleration (src/physics_two_body.hpp:208-218)
+            double dx1 = x - prm.x1, dy1 = y - prm.y1;
+            double dx2 = x - prm.x2, dy2 = y - prm.y2;
+            double r21 = dx1 * dx1 + dy1 * dy1;
+            double r22 = dx2 * dx2 + dy2 * dy2;
+            double g1 = -1.0 * prm.m1 / pow(r21 + prm.s2, 1.5);
+            double g2 = -1.0 * prm.m2 / pow(r22 + prm.s2, 1.5);
+            double ax1 = g1 * dx1, ay1 = g1 * dy1;
+            double ax2 = g2 * dx2, ay2 = g2 * dy2;
+            // coriolis_term :67-71, centrifugal_term :73-76 (from pc)
+            double om = prm.omega;
+            double corx = 2.0 * om * pc[2], cory = 2.0 * om * (-pc[1]);
+            double cenx = om * om * x, ceny = om * om * y;
+            // sink_rate_field :57-65
+            double r61 = pow((0.0 + dx1 * dx1) + dy1 * dy1, 3.0);
+            double r62 = pow((0.0 + dx2 * dx2) + dy2 * dy2, 3.0);
+            double sr1 = prm.sink_rate * exp(-r61 / prm.s6);
+            double sr2 = prm.sink_rate * exp(-r62 / prm.s6);
+            // buffer_rate_field :47-55
+            double r = sqrt((0.0 + x * x) + y * y);
+            double yb = (r - prm.R) / prm.buffer_scale;
+            double br = 0.5 * prm.buffer_rate * (1.0 + tanh(yb));
+            // initial_conserved_array :182-191 via initial_primitive :87-99
+            double ph = -1.0 * 1.0 / sqrt((x * x + y * y) + prm.s2);
+            double r0 = sqrt(x * x + y * y);
+            double vp = sqrt(-ph) - om * r0;
+            double ui[3] = { 1.0, 1.0 * (vp * (-y / r0)), 1.0 * (vp * (x / r0)) };
+            double sig = pc[0];
+            double sg1[3] = { 0.0 / 1.0, sig * ax1, sig * ay1 };
+            double sg2[3] = { 0.0 / 1.0, sig * ax2, sig * ay2 };
+            double sf1[3] = { 0.0 / 1.0, sig * cenx, sig * ceny };
+            double sf2[3] = { 0.0 / 1.0, sig * corx, sig * cory };
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                double ss1 = -u[q] * sr1;
+                double ss2 = -u[q] * sr2;
+                double sb = -(u[q] - ui[q]) * br;
+                double src = sg1[q] + sg2[q] + ss1 + ss2 + sf1[q] + sf2[q] + sb;
+                out[q] = u[q] - (dfx[q] + dfy[q]) * dt / dl + src * dt;
+            }
+        } else {
+            float xf = float(x), yf = float(y);
+            float s2 = float(prm.s2);
+            float dx1 = xf - float(prm.x1), dy1 = yf - float(prm.y1);
+            float dx2 = xf - float(prm.x2), dy2 = yf - float(prm.y2);
+            float q1 = fmaf(dx1, dx1, dy1 * dy1), q2 = fmaf(dx2, dx2, dy2 * dy2);
+            float i1 = __builtin_amdgcn_rsqf(q1 + s2), i2 = __builtin_amdgcn_rsqf(q2 + s2);
+            float g1 = -float(prm.m1) * i1 * i1 * i1;
+            float g2 = -float(prm.m2) * i2 * i2 * i2;
+            float om = float(prm.omega);
+            float sig = pc[0];
+            float ax = fmaf(g1, dx1, fmaf(g2, dx2, fmaf(om * om, xf, 2.0f * om * pc[2])));
+            float ay = fmaf(g1, dy1, fmaf(g2, dy2, fmaf(om * om, yf, -2.0f * om * pc[1])));
+            float is6 = float(1.0 / prm.s6);
+            float sr = float(prm.sink_rate) * (__expf(-q1 * q1 * q1 * is6) + __expf(-q2 * q2 * q2 * is6));
+            float rr = sqrtf(fmaf(xf, xf, yf * yf));
+            float yb = (rr - float(prm.R)) * float(1.0 / prm.buffer_scale);
+            // 0.5 * rate * (1 + tanh(y)) = rate / (1 + exp(-2y))
+            float br = float(prm.buffer_rate) * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * yb));
+            float ph = __builtin_amdgcn_rsqf(fmaf(xf, xf, fmaf(yf, yf, s2)));
+            float ir0 = __builtin_amdgcn_rsqf(fmaf(xf, xf, yf * yf));
+            float vp = sqrtf(ph) - om * rr;
+            float ui[3] = { 1.0f, vp * (-yf * ir0), vp * (xf * ir0) };
+            float sa[3] = { 0.0f, sig * ax, sig * ay };
+            double dtdl = dt / dl;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                float src = sa[q] - float(u[q]) * sr - float(u[q] - double(ui[q])) * br;
+                out[q] = u[q] - (dfx[q] + dfy[q]) * dtdl + double(src) * dt;
+            }
+        }
+    }
+};
+
+} // namespace m3p

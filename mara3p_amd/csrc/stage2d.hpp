@@ -1,0 +1,236 @@
+// The fused 2D stage kernel: guard fill + recover + PLM + Riemann + source
+// terms + conserved update (+ RK combine) in one pass per RK stage.
+//
+// Work decomposition ("tile march").  A workgroup of TJ threads owns TJ
+// adjacent columns (the contiguous axis) of one block and marches down H
+// rows.  Each thread keeps a three-row register window of primitives for the
+// x (row-axis) stencil, so every x face flux is computed once and carried to
+// the next row.  The y (column-axis) stencil goes through three LDS rows
+// (primitives, slopes, fluxes); the two halo columns on each side and the
+// tile's right-edge y flux for all H rows are computed once, up front, by a
+// halo pre-pass, so no lane does redundant per-row work.
+//
+// Guard zones are never materialised: a zone outside the block is read
+// directly from the neighbouring block through the neighbour table (the
+// periodic next_on/prev_on wrap of src/mesh_block_index.hpp:334-335 when the
+// table is built for a uniform mesh), which is amr::extend_block
+// (src/mesh_amr.hpp:360-393) fused into the load.
+#pragma once
+#include "schemes2d.hpp"
+
+namespace m3p {
+
+// Conserved state of a whole mesh: [nblk][NC][n][n], neighbour table
+// [nblk][4] = {prev on axis 0, next on axis 0, prev on axis 1, next on axis 1}.
+template<class S>
+struct MeshLoader {
+    const double* u;
+    const int* nbr;
+    int n;
+
+    __device__ __forceinline__ const double* zone(int b, int r, int c) const
+    {
+        if (r < 0)        { b = nbr[4 * b + 0]; r += n; }
+        else if (r >= n)  { b = nbr[4 * b + 1]; r -= n; }
+        if (c < 0)        { b = nbr[4 * b + 2]; c += n; }
+        else if (c >= n)  { b = nbr[4 * b + 3]; c -= n; }
+        return u + ((size_t)b * S::NC * n + r) * n + c;
+    }
+    __device__ __forceinline__ void cons(int b, int r, int c, double* uu) const
+    {
+        const double* z = zone(b, r, c);
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q) uu[q] = z[(size_t)q * n * n];
+    }
+    // Returns false when the recovery flagged the zone (negative density).
+    __device__ __forceinline__ bool prim(int b, int r, int c, const typename S::Params& prm,
+                                         typename S::Real* p) const
+    {
+        double uu[S::NC];
+        cons(b, r, c, uu);
+        return S::recover(uu, prm, p);
+    }
+};
+
+// One block given as the reference's per-block arguments: uc [NC][n][n]
+// conserved and pe [NC][n+4][n+4] primitives with two guard zones.
+template<class S>
+struct ExtendedLoader {
+    const double* uc;
+    const double* pe;
+    int n;
+
+    __device__ __forceinline__ void cons(int, int r, int c, double* uu) const
+    {
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q) uu[q] = uc[((size_t)q * n + r) * n + c];
+    }
+    __device__ __forceinline__ bool prim(int, int r, int c, const typename S::Params&,
+                                         typename S::Real* p) const
+    {
+        const int e = n + 4;
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q) p[q] = typename S::Real(pe[((size_t)q * e + (r + 2)) * e + (c + 2)]);
+        return true;
+    }
+};
+
+struct StageArgs {
+    const double* u0;      // RK base state (same layout as out) or nullptr
+    double* out;           // [nblk][NC][n][n]
+    const double* dt_dev;  // device-resident dt (graph replay) or nullptr
+    double dt;
+    double dl;
+    double rk_b;           // out = u0 * rk_b + L(u) * (1 - rk_b) when u0 != nullptr
+    const int* bcoord;     // [nblk][2] block coordinates, or nullptr (single block at bi0,bj0)
+    int bi0, bj0;
+    int n;                 // block size
+    int nblk;
+    int* err;
+};
+
+template<class S, class Ld, int H, int TJ>
+__global__ void __launch_bounds__(TJ)
+stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
+{
+    constexpr int NC = S::NC;
+    using R = typename S::Real;
+
+    const int n = a.n;
+    const int tiles_i = n / H, tiles_j = n / TJ;
+    const int per_block = tiles_i * tiles_j;
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    TileCtx tc;
+    tc.b = w / per_block;
+    const int rem = w % per_block;
+    tc.k0 = (rem / tiles_j) * H;
+    tc.c0 = (rem % tiles_j) * TJ;
+    tc.n = n;
+    if (a.bcoord) { tc.bi = a.bcoord[2 * tc.b]; tc.bj = a.bcoord[2 * tc.b + 1]; }
+    else          { tc.bi = a.bi0; tc.bj = a.bj0; }
+
+    const int t = threadIdx.x;
+    const int c = tc.c0 + t;
+    const int b = tc.b;
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+
+    __shared__ typename S::template TileData<H, TJ> td;
+    __shared__ R hraw[7][NC][H];     // pre-pass primitives at 7 edge columns
+    __shared__ R hG[NC][H];          // y slope at column c0-1
+    __shared__ R hF[NC][H];          // y flux at face c0+TJ
+    __shared__ R rowP[NC][TJ + 2];   // [0] = c0-1, [t+1] = c0+t, [TJ+1] = c0+TJ
+    __shared__ R rowG[NC][TJ + 1];   // [0] = c0-1, [t+1] = c0+t
+    __shared__ R rowF[NC][TJ + 1];   // [t] = face c0+t, [TJ] = face c0+TJ
+
+    S::setup(td, prm, tc);
+    bool ok = true;
+
+    // ---- halo pre-pass --------------------------------------------------
+    for (int idx = t; idx < 7 * H; idx += TJ) {
+        const int r = idx / 7, q = idx % 7;
+        const int cols[7] = { tc.c0 - 2, tc.c0 - 1, tc.c0, tc.c0 + TJ - 2, tc.c0 + TJ - 1, tc.c0 + TJ, tc.c0 + TJ + 1 };
+        R p[NC];
+        ok &= ld.prim(b, tc.k0 + r, cols[q], prm, p);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) hraw[q][m][r] = p[m];
+    }
+    __syncthreads();
+    const R theta = R(S::theta(prm));
+    for (int r = t; r < H; r += TJ) {
+        R pl[NC], pr[NC], f[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            hG[m][r] = plm(hraw[0][m][r], hraw[1][m][r], hraw[2][m][r], theta);
+            R g1 = plm(hraw[3][m][r], hraw[4][m][r], hraw[5][m][r], theta);
+            R g2 = plm(hraw[4][m][r], hraw[5][m][r], hraw[6][m][r], theta);
+            pl[m] = hraw[4][m][r] + R(0.5) * g1;
+            pr[m] = hraw[5][m][r] - R(0.5) * g2;
+        }
+        S::template face<2>(pl, pr, prm, td, tc, tc.k0 + r, tc.c0 + TJ, f);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) hF[m][r] = f[m];
+    }
+
+    // ---- x warm-up: rows k0-2 .. k0+1, flux at face k0 ------------------
+    R Pa[NC], Pb[NC], ga[NC], Fxp[NC];
+    {
+        R Pm2[NC], Pm1[NC], gm[NC], pl[NC], pr[NC];
+        ok &= ld.prim(b, tc.k0 - 2, c, prm, Pm2);
+        ok &= ld.prim(b, tc.k0 - 1, c, prm, Pm1);
+        ok &= ld.prim(b, tc.k0, c, prm, Pa);
+        ok &= ld.prim(b, tc.k0 + 1, c, prm, Pb);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            gm[m] = plm(Pm2[m], Pm1[m], Pa[m], theta);
+            ga[m] = plm(Pm1[m], Pa[m], Pb[m], theta);
+            pl[m] = Pm1[m] + R(0.5) * gm[m];
+            pr[m] = Pa[m] - R(0.5) * ga[m];
+        }
+        S::template face<1>(pl, pr, prm, td, tc, tc.k0, c, Fxp);
+    }
+    __syncthreads();
+
+    // ---- march ------------------------------------------------------------
+    for (int kk = 0; kk < H; ++kk) {
+        const int k = tc.k0 + kk;
+        R Pc[NC], gb[NC], pl[NC], pr[NC], Fxn[NC], gy[NC], Fy[NC];
+        ok &= ld.prim(b, k + 2, c, prm, Pc);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            gb[m] = plm(Pa[m], Pb[m], Pc[m], theta);
+            pl[m] = Pa[m] + R(0.5) * ga[m];
+            pr[m] = Pb[m] - R(0.5) * gb[m];
+        }
+        S::template face<1>(pl, pr, prm, td, tc, k + 1, c, Fxn);
+
+#pragma unroll
+        for (int m = 0; m < NC; ++m) rowP[m][t + 1] = Pa[m];
+        if (t == 0) {
+#pragma unroll
+            for (int m = 0; m < NC; ++m) { rowP[m][0] = hraw[1][m][kk]; rowG[m][0] = hG[m][kk]; }
+        }
+        if (t == TJ - 1) {
+#pragma unroll
+            for (int m = 0; m < NC; ++m) { rowP[m][TJ + 1] = hraw[5][m][kk]; rowF[m][TJ] = hF[m][kk]; }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            gy[m] = plm(rowP[m][t], Pa[m], rowP[m][t + 2], theta);
+            rowG[m][t + 1] = gy[m];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            pl[m] = rowP[m][t] + R(0.5) * rowG[m][t];
+            pr[m] = Pa[m] - R(0.5) * gy[m];
+        }
+        S::template face<2>(pl, pr, prm, td, tc, k, c, Fy);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) rowF[m][t] = Fy[m];
+        __syncthreads();
+
+        double dfx[NC], dfy[NC], uu[NC], res[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            dfx[m] = double(Fxn[m]) - double(Fxp[m]);
+            dfy[m] = double(rowF[m][t + 1]) - double(Fy[m]);
+        }
+        ld.cons(b, k, c, uu);
+        S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, a.dl, res);
+        double* o = a.out + ((size_t)b * NC * n + k) * n + c;
+        if (a.u0) {
+            const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
+#pragma unroll
+            for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = z0[(size_t)m * n * n] * a.rk_b + res[m] * (1.0 - a.rk_b);
+        } else {
+#pragma unroll
+            for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = res[m];
+        }
+#pragma unroll
+        for (int m = 0; m < NC; ++m) { Pa[m] = Pb[m]; Pb[m] = Pc[m]; ga[m] = gb[m]; Fxp[m] = Fxn[m]; }
+    }
+    if (!ok) report_error(a.err, 1, b, 0);
+}
+
+} // namespace m3p

@@ -1,0 +1,146 @@
+"""Seeded synthetic initial conditions for the benchmark configurations.
+
+All randomness comes from a counter-based generator keyed by (seed, global
+zone index), so the initial state does not depend on the block decomposition
+or on the order of evaluation.  Arrays are returned in the device layout
+[nblk, nc, n, n] (2D) or [nblk, nc, n] (1D), conserved variables.
+"""
+import math
+
+import numpy as np
+
+from .mesh import UniformMesh2D
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & _M64
+    z = x
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed, index, stream=0):
+    """Uniform doubles in (0, 1) keyed by (seed, stream, index)."""
+    with np.errstate(over="ignore"):
+        key = _splitmix64(np.uint64(seed) * np.uint64(0x100000001B3) + np.uint64(stream))
+        z = _splitmix64(np.asarray(index, dtype=np.uint64) ^ key)
+    return ((z >> np.uint64(11)).astype(np.float64) + 0.5) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed, index, stream=0):
+    """Standard normals by Box-Muller from two counter streams."""
+    u1 = uniform01(seed, index, 2 * stream)
+    u2 = uniform01(seed, index, 2 * stream + 1)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+def _global_index_2d(mesh):
+    """Global zone index (row-major over the whole level) for every zone, [nblk, n, n]."""
+    n, nb = mesh.n, mesh.nb
+    ij = mesh.block_coordinates()
+    i = ij[:, 0, None, None] * n + np.arange(n)[None, :, None]
+    j = ij[:, 1, None, None] * n + np.arange(n)[None, None, :]
+    return (i * (nb * n) + j).astype(np.uint64)
+
+
+def cartesian_cell_centres(mesh, domain_size=1.0):
+    """mesh::cartesian_2d::geometry_t cell coordinates (src/mesh_geometry.cpp:114-146):
+    block extents on [-L/2, L/2], linspace vertices, adjacent means.  [nblk, n, n] each."""
+    n = mesh.n
+    dl = float(1 << mesh.depth)
+    ij = mesh.block_coordinates().astype(np.float64)
+    x0 = domain_size * (-0.5 + 1.0 * ij[:, 0] / dl)
+    x1 = domain_size * (-0.5 + 1.0 * (ij[:, 0] + 1) / dl)
+    y0 = domain_size * (-0.5 + 1.0 * ij[:, 1] / dl)
+    y1 = domain_size * (-0.5 + 1.0 * (ij[:, 1] + 1) / dl)
+    k = np.arange(n + 1, dtype=np.float64)
+    xv = x0[:, None] + (x1 - x0)[:, None] * k[None, :] / float(n)
+    yv = y0[:, None] + (y1 - y0)[:, None] * k[None, :] / float(n)
+    xc = 0.5 * (xv[:, :-1] + xv[:, 1:])
+    yc = 0.5 * (yv[:, :-1] + yv[:, 1:])
+    return np.broadcast_to(xc[:, :, None], (mesh.nblk, n, n)), np.broadcast_to(yc[:, None, :], (mesh.nblk, n, n))
+
+
+def euler_conserved(rho, vx, vy, vz, p, gamma):
+    """euler::conserved_density (src/physics_euler.hpp:175-183), vectorised, stacked on axis 1."""
+    ls = ((0.0 + vx * vx) + vy * vy) + vz * vz
+    return np.stack([rho, rho * vx, rho * vy, rho * vz, rho * ls * 0.5 + p / (gamma - 1.0)], axis=1)
+
+
+def kelvin_helmholtz(depth=4, block_size=256, seed=1, gamma=5.0 / 3.0):
+    """2D Kelvin-Helmholtz on [-1/2, 1/2]^2, periodic: rho = 2 in |y| < 1/4 else 1,
+    vx = +-1/2, p = 2.5, vy = 0.01 sin(4 pi x) + U(-1e-3, 1e-3)."""
+    mesh = UniformMesh2D(depth, block_size)
+    x, y = cartesian_cell_centres(mesh)
+    inner = np.abs(y) < 0.25
+    rho = np.where(inner, 2.0, 1.0)
+    vx = np.where(inner, 0.5, -0.5)
+    noise = 2e-3 * uniform01(seed, _global_index_2d(mesh)) - 1e-3
+    vy = 0.01 * np.sin(4.0 * math.pi * x) + noise
+    vz = np.zeros_like(rho)
+    p = np.full_like(rho, 2.5)
+    return mesh, euler_conserved(rho, vx, vy, vz, p, gamma)
+
+
+def minidisk_cell_centres(data, mesh):
+    """problems/minidisk_scheme.cpp:101-116,153-165: [nblk, n, n] x and y."""
+    n = mesh.n
+    dl = float(1 << mesh.depth)
+    R = data.domain_radius
+    ij = mesh.block_coordinates().astype(np.int64)
+
+    def verts(b):
+        x0 = R * (-1 + (2 * b) / dl)
+        x1 = R * (-1 + (2 * (b + 1)) / dl)
+        k = np.arange(n + 1, dtype=np.float64)
+        return x0[:, None] + (x1 - x0)[:, None] * k[None, :] / float(n)
+
+    xv, yv = verts(ij[:, 0]), verts(ij[:, 1])
+    xc = 0.5 * (xv[:, :-1] + xv[:, 1:])
+    yc = 0.5 * (yv[:, :-1] + yv[:, 1:])
+    return np.broadcast_to(xc[:, :, None], (mesh.nblk, n, n)), np.broadcast_to(yc[:, None, :], (mesh.nblk, n, n))
+
+
+def minidisk(data, seed=1, noise=0.01):
+    """Locally isothermal binary minidisk: Keplerian initial_primitive
+    (problems/minidisk_scheme.cpp:87-99) with surface density
+    Sigma = exp(noise * z), z ~ N(0, 1) per zone (log-normal, median 1, no
+    mean correction)."""
+    mesh = UniformMesh2D(data.depth, data.block_size)
+    x, y = minidisk_cell_centres(data, mesh)
+    rs = data.softening_length
+    r2 = x * x + y * y
+    ph = -1.0 / np.sqrt(r2 + rs ** 2.0)
+    r0 = np.sqrt(r2)
+    vp = np.sqrt(-ph) - data.omega_frame * r0
+    vx = vp * (-y / r0)
+    vy = vp * (x / r0)
+    sigma = np.exp(noise * normal(seed, _global_index_2d(mesh))) if noise else np.ones_like(x)
+    u = np.stack([sigma, sigma * vx, sigma * vy], axis=1)
+    return mesh, np.ascontiguousarray(u)
+
+
+def minidisk_dt(data, time):
+    """problems/minidisk.cpp:202-206: dt = dx_min / sqrt(G 0.5 / r_soft) * cfl * ramp(t)."""
+    vmax = math.sqrt(1.0 * 0.5 / data.softening_length)
+    cfl = data.cfl * min(1.0, 0.01 + time / 0.05)
+    dx = 2.0 * data.domain_radius / float(data.block_size) / float(1 << data.depth)
+    return dx / vmax * cfl
+
+
+def sod(nblk=16, block_size=64, gamma=1.4):
+    """Sod shock tube on [0, 1] (left (1, 0, 1), right (0.125, 0, 0.1)), outflow.
+    The reference 1D geometry spans [-1/2, 1/2] (src/mesh_geometry.cpp:72-82);
+    the interface is its centre."""
+    n = block_size
+    N = nblk * n
+    xc = (np.arange(N) + 0.5) / N - 0.5
+    left = xc < 0.0
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    z = np.zeros(N)
+    u = euler_conserved(rho, z, z, z, p, gamma)        # [N, 5]
+    return np.ascontiguousarray(u.reshape(nblk, n, 5).transpose(0, 2, 1))

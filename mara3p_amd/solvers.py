@@ -1,0 +1,139 @@
+"""Device-resident solvers: the whole mesh lives in HBM for the run; each RK2
+step is two fused stage launches (control::advance_runge_kutta with
+weighted_sum_tree, src/app_control.hpp:169-195, src/mesh_amr.hpp:455-467).
+
+PyTorch provides device memory and the stream; all arithmetic is in the HIP
+library.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, MinidiskData, PRECISIONS
+from .mesh import ChainMesh1D, UniformMesh2D
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class _StateSolver:
+    nc = None
+
+    def _alloc(self, shape, device):
+        self.device = torch.device(device)
+        self.u = torch.zeros(shape, dtype=torch.float64, device=self.device)
+        self.u1 = torch.empty_like(self.u)
+        self.u2 = torch.empty_like(self.u)
+        self.totals = torch.zeros(self.nc, dtype=torch.float64, device=self.device)
+
+    def set_state(self, u):
+        self.u.copy_(torch.as_tensor(np.ascontiguousarray(u), dtype=torch.float64))
+
+    def state(self):
+        return self.u.cpu().numpy()
+
+    def conserved_totals(self):
+        zpb = int(np.prod(self.u.shape[2:]))
+        check(lib().m3p_conserved_totals(_ptr(self.u), self.u.shape[0], self.nc, zpb, _ptr(self.totals), _stream()))
+        return self.totals.cpu().numpy()
+
+    def check_errors(self):
+        _lib.check_kernel_errors(torch.cuda.current_stream().cuda_stream)
+
+
+class Euler2DSolver(_StateSolver):
+    """euler2d on a uniform periodic quad-tree level (module_euler.cpp:263-323)."""
+    nc = 5
+
+    def __init__(self, depth, block_size, gamma=5.0 / 3.0, plm_theta=1.5, precision="fp64",
+                 hllc=False, domain_size=1.0, device="cuda"):
+        self.mesh = UniformMesh2D(depth, block_size)
+        self.gamma, self.theta, self.hllc = gamma, plm_theta, int(hllc)
+        self.precision = PRECISIONS[precision]
+        # cell_spacing, src/mesh_geometry.cpp:168-171
+        self.dl = domain_size / float(block_size) / float(1 << depth)
+        self._alloc((self.mesh.nblk, 5, block_size, block_size), device)
+        self.nbr = torch.as_tensor(self.mesh.neighbours(), device=self.device)
+        self.dt_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
+
+    def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None):
+        check(lib().m3p_euler2d_stage(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), self.mesh.nblk, self.mesh.n,
+                                      dt, _ptr(dt_dev), self.dl, rk_b, self.theta, self.gamma, self.hllc,
+                                      self.precision, _stream()))
+
+    def step(self, dt):
+        self.stage(self.u, None, self.u1, dt)
+        self.stage(self.u1, self.u, self.u2, dt)
+        self.u, self.u2 = self.u2, self.u
+
+    def cfl_dt(self, cfl):
+        """CFL dt from the current state into self.dt_dev (device) and returned."""
+        zpb = self.mesh.n * self.mesh.n
+        check(lib().m3p_euler_cfl_dt(_ptr(self.u), self.mesh.nblk, zpb, self.gamma, cfl, self.dl,
+                                     _ptr(self.dt_dev), _stream()))
+        return self.dt_dev
+
+    def step_cfl(self, cfl):
+        """One RK2 step with dt = cfl dl / max(|v| + c_s) kept on the device."""
+        dt_dev = self.cfl_dt(cfl)
+        self.stage(self.u, None, self.u1, 0.0, dt_dev=dt_dev)
+        self.stage(self.u1, self.u, self.u2, 0.0, dt_dev=dt_dev)
+        self.u, self.u2 = self.u2, self.u
+
+
+class MinidiskSolver(_StateSolver):
+    """minidisk::updated_conserved on every block, RK2 (problems/minidisk_scheme.cpp:294-355)."""
+    nc = 3
+
+    def __init__(self, data: MinidiskData, precision="fp64", device="cuda"):
+        self.data = data
+        self.mesh = UniformMesh2D(data.depth, data.block_size)
+        self.precision = PRECISIONS[precision]
+        n = data.block_size
+        self._alloc((self.mesh.nblk, 3, n, n), device)
+        self.nbr = torch.as_tensor(self.mesh.neighbours(), device=self.device)
+        self.bcoord = torch.as_tensor(self.mesh.block_coordinates(), device=self.device)
+        self.time = 0.0
+
+    def stage(self, u, u0, out, time, dt, rk_b=0.5):
+        check(lib().m3p_minidisk_stage(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), _ptr(self.bcoord),
+                                       self.mesh.nblk, time, dt, rk_b, ctypes.byref(self.data),
+                                       self.precision, _stream()))
+
+    def step(self, dt):
+        t = self.time
+        self.stage(self.u, None, self.u1, t, dt)
+        self.stage(self.u1, self.u, self.u2, t + dt, dt)
+        self.u, self.u2 = self.u2, self.u
+        # solution time combines like the state: s*b + t*(1-b) (src/module_euler.cpp:357)
+        self.time = t * 0.5 + (t + dt) * (1.0 - 0.5)
+
+
+class Euler1DSolver(_StateSolver):
+    """euler1d on a chain of blocks with outflow ends (module_euler.cpp:107-161, mara.cpp:123-143)."""
+    nc = 5
+
+    def __init__(self, nblk, block_size, gamma=1.4, plm_theta=1.5, precision="fp64",
+                 domain_size=1.0, outflow=True, device="cuda"):
+        self.mesh = ChainMesh1D(nblk, block_size, outflow)
+        self.gamma, self.theta = gamma, plm_theta
+        self.precision = PRECISIONS[precision]
+        self.dl = domain_size / float(nblk * block_size)
+        self._alloc((nblk, 5, block_size), device)
+        self.nbr = torch.as_tensor(self.mesh.neighbours(), device=self.device)
+
+    def stage(self, u, u0, out, dt, rk_b=0.5):
+        check(lib().m3p_euler1d_stage(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), self.mesh.nblk, self.mesh.n,
+                                      dt, self.dl, rk_b, self.theta, self.gamma, self.precision, _stream()))
+
+    def step(self, dt):
+        self.stage(self.u, None, self.u1, dt)
+        self.stage(self.u1, self.u, self.u2, dt)
+        self.u, self.u2 = self.u2, self.u

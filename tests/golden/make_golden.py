@@ -1,0 +1,143 @@
+"""Write golden vectors from the reference's own hot-path code.
+
+Runs in the build container only: it calls oracle/_ref/libmara_ref.so, which
+oracle/Makefile compiles from the reference sources where they lie
+(problems/minidisk_scheme.cpp plus the header-only physics).  The outputs are
+small .npz files of inputs and expected outputs; the reference itself never
+travels.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle  # noqa: E402
+from oracle import P  # noqa: E402
+
+SEED = 20261202
+
+
+def plm_cases(rng):
+    ys, ths = [], []
+    for th in (1.0, 1.5, 2.0):
+        for _ in range(300):
+            y = rng.normal(size=3)
+            k = rng.integers(0, 6)
+            if k == 0:
+                y[1] = y[0]
+            elif k == 1:
+                y[2] = y[1]
+            elif k == 2:
+                y[:] = y[0]
+            elif k == 3:
+                y[1] = 0.0
+                y[0] = -0.0
+            ys.append(y)
+            ths.append(th)
+    ys = np.array(ys)
+    ths = np.array(ths)
+    out = np.array([oracle.ref().ref_plm(*y, t) for y, t in zip(ys, ths)])
+    return dict(y=ys, theta=ths, g=out)
+
+
+def euler_cases(rng, m=400):
+    pl = np.column_stack([rng.uniform(0.1, 3, m), rng.normal(0, 2, (m, 3)), rng.uniform(0.05, 3, m)])
+    pr = np.column_stack([rng.uniform(0.1, 3, m), rng.normal(0, 2, (m, 3)), rng.uniform(0.05, 3, m)])
+    axis = rng.integers(1, 4, m).astype(np.int32)
+    gamma = rng.choice([1.4, 5.0 / 3.0], m)
+    f = np.zeros((m, 5))
+    flux = np.zeros((m, 5))
+    u = np.zeros((m, 5))
+    p2 = np.zeros((m, 5))
+    for i in range(m):
+        a, b, o = np.ascontiguousarray(pl[i]), np.ascontiguousarray(pr[i]), np.zeros(5)
+        oracle.ref().ref_euler_riemann_hlle(P(a), P(b), int(axis[i]), gamma[i], P(o))
+        f[i] = o
+        o = np.zeros(5)
+        oracle.ref().ref_euler_flux(P(a), int(axis[i]), gamma[i], P(o))
+        flux[i] = o
+        o = np.zeros(5)
+        oracle.ref().ref_euler_conserved_density(P(a), gamma[i], P(o))
+        u[i] = o
+        o = np.zeros(5)
+        oracle.ref().ref_euler_recover_primitive(P(np.ascontiguousarray(u[i])), gamma[i], P(o))
+        p2[i] = o
+    return dict(pl=pl, pr=pr, axis=axis, gamma=gamma, hlle=f, flux=flux, cons=u, recovered=p2)
+
+
+def iso_cases(rng, m=400):
+    pl = np.column_stack([rng.uniform(0.1, 3, m), rng.normal(0, 2, (m, 2))])
+    pr = np.column_stack([rng.uniform(0.1, 3, m), rng.normal(0, 2, (m, 2))])
+    axis = rng.integers(1, 3, m).astype(np.int32)
+    cs2 = rng.uniform(1e-3, 2.0, m)
+    f = np.zeros((m, 3))
+    for i in range(m):
+        o = np.zeros(3)
+        oracle.ref().ref_iso2d_riemann_hlle(P(np.ascontiguousarray(pl[i])), P(np.ascontiguousarray(pr[i])),
+                                            cs2[i], int(axis[i]), P(o))
+        f[i] = o
+    return dict(pl=pl, pr=pr, axis=axis, cs2=cs2, hlle=f)
+
+
+def two_body_cases(rng):
+    rows, out = [], []
+    for q in (1.0, 0.5, 0.1):
+        for e in (0.0, 0.3, 0.7):
+            for t in (0.0, 0.37, 3.1, 25.0):
+                for om in (0.0, 1.0):
+                    st = np.zeros(10)
+                    oracle.ref().ref_two_body_state(1.0, 1.0, q, e, t, om, P(st))
+                    rows.append((q, e, t, om))
+                    out.append(st)
+    return dict(args=np.array(rows), state=np.array(out))
+
+
+def minidisk_cases(rng):
+    # solver data: {block_size, depth, omega_frame, mach, nu, domain_radius, softening,
+    #               eccentricity, mass_ratio, sink_radius, sink_rate, buffer_rate, buffer_scale, cfl}
+    n = 16
+    sds, ucs, pes, outs, meta = [], [], [], [], []
+    configs = [
+        (2, 0, 0, 0.0, 1e-3, dict(domain_radius=1.5, softening=0.05, sink_radius=0.05, q=1.0, e=0.0, om=1.0)),
+        (2, 1, 2, 0.7, 2e-3, dict(domain_radius=1.5, softening=0.05, sink_radius=0.05, q=1.0, e=0.0, om=1.0)),
+        (2, 3, 1, 2.3, 5e-4, dict(domain_radius=2.0, softening=0.02, sink_radius=0.1, q=0.5, e=0.2, om=0.5)),
+        (1, 1, 0, 0.1, 1e-3, dict(domain_radius=8.0, softening=0.05, sink_radius=0.05, q=1.0, e=0.0, om=1.0)),
+    ]
+    for level, bi, bj, time, dt, c in configs:
+        sd = np.array([n, level, c["om"], 10.0, 0.0, c["domain_radius"], c["softening"], c["e"], c["q"],
+                       c["sink_radius"], 1e3, 1e2, 0.2, 0.3])
+        ui = np.zeros((n, n, 3))
+        oracle.ref().ref_minidisk_initial_conserved(P(sd), level, bi, bj, P(ui))
+        uc = np.ascontiguousarray(ui * np.exp(0.05 * rng.normal(size=(n, n, 1))))
+        pe = np.zeros((n + 4, n + 4, 3))
+        pe[..., 0] = rng.uniform(0.5, 1.5, (n + 4, n + 4))
+        pe[..., 1:] = rng.normal(0, 1, (n + 4, n + 4, 2))
+        out = np.zeros((n, n, 3))
+        oracle.ref().ref_minidisk_updated_conserved(P(uc), P(pe), P(sd), level, bi, bj, time, dt, P(out))
+        sds.append(sd)
+        ucs.append(uc)
+        pes.append(pe)
+        outs.append(out)
+        meta.append((level, bi, bj, time, dt))
+    return dict(sd=np.array(sds), uc=np.array(ucs), pe=np.array(pes), out=np.array(outs), meta=np.array(meta))
+
+
+def main():
+    if not oracle.ref_available():
+        raise SystemExit("oracle/_ref/libmara_ref.so is not built (make -C oracle ref)")
+    rng = np.random.default_rng(SEED)
+    np.savez_compressed(os.path.join(HERE, "plm.npz"), **plm_cases(rng))
+    np.savez_compressed(os.path.join(HERE, "euler_pointwise.npz"), **euler_cases(rng))
+    np.savez_compressed(os.path.join(HERE, "iso2d_hlle.npz"), **iso_cases(rng))
+    np.savez_compressed(os.path.join(HERE, "two_body.npz"), **two_body_cases(rng))
+    np.savez_compressed(os.path.join(HERE, "minidisk_block.npz"), **minidisk_cases(rng))
+    print("golden vectors written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

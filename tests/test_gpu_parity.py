@@ -1,0 +1,296 @@
+"""GPU parity: the HIP kernels, called through the C ABI, against the oracle.
+
+FP64 Euler (HLLE and HLLC) must equal the oracle exactly (== on every value).
+FP64 minidisk must agree to 1e-12 relative per zone per step (normalised by
+the component's max |u|): its sources call exp/pow/tanh, whose device (ocml)
+and host (glibc) implementations may differ by an ulp.  The mixed kernel is
+checked against FP64 at a stated tolerance and for exact conservation.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a HIP device")]
+
+import oracle  # noqa: E402
+from oracle import OrcMinidisk, P, orc  # noqa: E402
+
+
+def _m3p():
+    import mara3p_amd
+    return mara3p_amd
+
+
+def c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def dev(a):
+    return torch.as_tensor(c(a), device="cuda")
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def rel_err(got, ref):
+    """max over zones of |got - ref| / max|ref| per component (component axis = 1)."""
+    scale = np.abs(ref).max(axis=tuple(i for i in range(ref.ndim) if i != 1), keepdims=True)
+    return float((np.abs(got - ref) / np.maximum(scale, 1e-300)).max())
+
+
+def smooth_euler_block(n, rng, ext=True):
+    """Random smooth positive primitive field on an (n+4)^2 extended block, conserved interior."""
+    m = n + 4
+    x = np.arange(m)[:, None] / m
+    y = np.arange(m)[None, :] / m
+    k = rng.uniform(1, 3, 6)
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * (k[0] * x + k[1] * y)) + 0.05 * rng.random((m, m))
+    v1 = 0.4 * np.cos(2 * np.pi * k[2] * y) + 0.05 * rng.standard_normal((m, m))
+    v2 = 0.3 * np.sin(2 * np.pi * k[3] * x) + 0.05 * rng.standard_normal((m, m))
+    v3 = 0.1 * rng.standard_normal((m, m))
+    p = 1.0 + 0.2 * np.cos(2 * np.pi * (k[4] * x - k[5] * y)) + 0.05 * rng.random((m, m))
+    pe = np.stack([rho, v1, v2, v3, p], -1)
+    uc = np.zeros((n, n, 5))
+    for i in range(n):
+        for j in range(n):
+            orc().orc_euler_conserved_density(P(c(pe[i + 2, j + 2])), 1.4, P(uc[i, j]))
+    uc *= 1.0 + 1e-3 * rng.standard_normal((n, n, 1))
+    return c(uc), c(pe)
+
+
+# --------------------------------------------------------------------------- per-block API
+@pytest.mark.parametrize("n", [16, 64, 256])
+@pytest.mark.parametrize("hllc", [0, 1])
+def test_euler2d_updated_conserved_fp64_exact(n, hllc):
+    m3p = _m3p()
+    rng = np.random.default_rng(n + 10 * hllc)
+    uc, pe = smooth_euler_block(n, rng)
+    dt, dl = 2e-3, 1.0 / n
+    ref = np.zeros_like(uc)
+    orc().orc_euler2d_updated_conserved(P(uc), P(pe), n, n, dt, dl, 1.5, 1.4, hllc, P(ref))
+    out = torch.empty((5, n, n), dtype=torch.float64, device="cuda")
+    m3p.check(m3p.lib().m3p_euler2d_updated_conserved(ptr(dev(np.moveaxis(uc, -1, 0))), ptr(dev(np.moveaxis(pe, -1, 0))),
+                                                      ptr(out), n, dt, dl, 1.5, 1.4, hllc, m3p.M3P_FP64, stream()))
+    got = np.moveaxis(out.cpu().numpy(), 0, -1)
+    assert np.array_equal(got, ref)
+
+
+def test_euler1d_updated_conserved_fp64_exact():
+    m3p = _m3p()
+    rng = np.random.default_rng(3)
+    n = 64
+    for lo, hi in ((0, 0), (1, 0), (0, 1), (1, 1)):
+        pe = np.stack([1 + 0.3 * rng.random(n + 4), 0.3 * rng.standard_normal(n + 4), 0.1 * rng.standard_normal(n + 4),
+                       0 * rng.random(n + 4), 1 + 0.3 * rng.random(n + 4)], -1)
+        uc = np.zeros((n, 5))
+        for i in range(n):
+            orc().orc_euler_conserved_density(P(c(pe[i + 2])), 1.4, P(uc[i]))
+        uc, pe = c(uc), c(pe)
+        ref = np.zeros_like(uc)
+        orc().orc_euler1d_updated_conserved(P(uc), P(pe), n, 1e-3, 1.0 / n, 1.5, 1.4, lo, hi, P(ref))
+        out = torch.empty((5, n), dtype=torch.float64, device="cuda")
+        m3p.check(m3p.lib().m3p_euler1d_updated_conserved(ptr(dev(uc.T)), ptr(dev(pe.T)), ptr(out), n, 1e-3, 1.0 / n,
+                                                          1.5, 1.4, lo, hi, m3p.M3P_FP64, stream()))
+        assert np.array_equal(out.cpu().numpy().T, ref)
+
+
+@pytest.mark.parametrize("n,level,bi,bj", [(16, 2, 1, 2), (64, 1, 0, 1), (256, 4, 7, 9)])
+def test_minidisk_updated_conserved_fp64(n, level, bi, bj):
+    m3p = _m3p()
+    rng = np.random.default_rng(level)
+    data = m3p.MinidiskData.defaults(block_size=n, depth=level, domain_radius=8.0, softening_length=0.05,
+                                     sink_radius=0.05, eccentricity=0.1, mass_ratio=0.8)
+    sd = OrcMinidisk.from_data(data)
+    ui = np.zeros((n, n, 3))
+    orc().orc_minidisk_initial_conserved(ctypes.byref(sd), level, bi, bj, P(ui))
+    uc = c(ui * np.exp(0.05 * rng.standard_normal((n, n, 1))))
+    pe = np.zeros((n + 4, n + 4, 3))
+    pe[..., 0] = rng.uniform(0.5, 1.5, (n + 4, n + 4))
+    pe[..., 1:] = rng.standard_normal((n + 4, n + 4, 2))
+    pe = c(pe)
+    ref = np.zeros_like(uc)
+    orc().orc_minidisk_updated_conserved(P(uc), P(pe), ctypes.byref(sd), level, bi, bj, 0.9, 1e-3, P(ref))
+    out = torch.empty((3, n, n), dtype=torch.float64, device="cuda")
+    m3p.check(m3p.lib().m3p_minidisk_updated_conserved(ptr(dev(np.moveaxis(uc, -1, 0))), ptr(dev(np.moveaxis(pe, -1, 0))),
+                                                       ptr(out), level, bi, bj, 0.9, 1e-3, ctypes.byref(data),
+                                                       m3p.M3P_FP64, stream()))
+    got = np.moveaxis(out.cpu().numpy(), 0, -1)
+    err = rel_err(got[None].transpose(0, 3, 1, 2), ref[None].transpose(0, 3, 1, 2))
+    assert err <= 1e-12, err
+
+
+# --------------------------------------------------------------------------- whole-mesh stages
+@pytest.mark.parametrize("depth,n,hllc", [(1, 64, 0), (2, 32, 0), (1, 64, 1), (2, 256, 0)])
+def test_euler2d_mesh_rk2_fp64_exact(depth, n, hllc):
+    from mara3p_amd import problems
+    from mara3p_amd.mesh import to_device_layout, to_reference_layout
+    from mara3p_amd.solvers import Euler2DSolver
+    mesh, u0 = problems.kelvin_helmholtz(depth=depth, block_size=n, seed=depth + n)
+    s = Euler2DSolver(depth, n, gamma=5.0 / 3.0, plm_theta=1.5, hllc=bool(hllc))
+    s.set_state(u0)
+    dt = 0.4 * s.dl / 2.0
+    steps = 3 if n < 256 else 1
+    for _ in range(steps):
+        s.step(dt)
+    s.check_errors()
+    ref = to_reference_layout(u0).copy()
+    assert orc().orc_euler2d_mesh_rk2(P(ref), mesh.nb, n, dt, s.dl, 1.5, 5.0 / 3.0, hllc, steps, 8) == 0
+    assert np.array_equal(s.state(), to_device_layout(ref))
+
+
+def test_minidisk_mesh_rk2_fp64():
+    from mara3p_amd import MinidiskData, problems
+    from mara3p_amd.mesh import to_device_layout, to_reference_layout
+    from mara3p_amd.solvers import MinidiskSolver
+    data = MinidiskData.defaults(block_size=64, depth=2, domain_radius=2.0, softening_length=0.05, sink_radius=0.05)
+    mesh, u0 = problems.minidisk(data, seed=9)
+    s = MinidiskSolver(data)
+    s.set_state(u0)
+    ref = to_reference_layout(u0).copy()
+    t = 0.0
+    for _ in range(3):
+        dt = problems.minidisk_dt(data, t)
+        s.step(dt)
+        orc().orc_minidisk_mesh_rk2(P(ref), OrcMinidisk.from_data(data), t, dt, 1, 8)
+        t = t * 0.5 + (t + dt) * 0.5
+        err = rel_err(s.state(), to_device_layout(ref))
+        assert err <= 1e-12, err
+
+
+def test_euler1d_sod_mesh_fp64_exact():
+    from mara3p_amd import problems
+    from mara3p_amd.mesh import to_reference_layout
+    from mara3p_amd.solvers import Euler1DSolver
+    nblk, n = 16, 64
+    u0 = problems.sod(nblk, n, 1.4)
+    s = Euler1DSolver(nblk, n, gamma=1.4, plm_theta=1.5)
+    s.set_state(u0)
+    dt = 0.4 * s.dl / 2.2
+    for _ in range(40):
+        s.step(dt)
+    s.check_errors()
+    ref = c(u0.transpose(0, 2, 1))
+    orc().orc_euler1d_mesh_rk2(P(ref), nblk, n, dt, s.dl, 1.5, 1.4, 40, 1)
+    assert np.array_equal(s.state(), ref.transpose(0, 2, 1))
+
+
+# --------------------------------------------------------------------------- mixed precision
+def test_minidisk_mixed_tracks_fp64():
+    from mara3p_amd import MinidiskData, problems
+    from mara3p_amd.solvers import MinidiskSolver
+    data = MinidiskData.defaults(block_size=64, depth=2, domain_radius=2.0, softening_length=0.05, sink_radius=0.05)
+    _, u0 = problems.minidisk(data, seed=2)
+    a, b = MinidiskSolver(data, "fp64"), MinidiskSolver(data, "mixed")
+    a.set_state(u0)
+    b.set_state(u0)
+    t = 0.0
+    for _ in range(20):
+        dt = problems.minidisk_dt(data, t)
+        a.step(dt)
+        b.step(dt)
+        t = a.time
+    ua, ub = a.state(), b.state()
+    l1 = np.mean(np.abs(ua[:, 0] - ub[:, 0]))
+    assert l1 < 1e-5, l1
+    assert rel_err(ub, ua) < 1e-3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_minidisk_mass_conservation(precision):
+    """With sink and buffer rates zero the mass update is a pure flux difference:
+    total mass is conserved to FP64 round-off (<= 1e-13 relative) in both kernels."""
+    from mara3p_amd import MinidiskData, problems
+    from mara3p_amd.solvers import MinidiskSolver
+    data = MinidiskData.defaults(block_size=64, depth=2, domain_radius=2.0, softening_length=0.05,
+                                 sink_radius=0.05, sink_rate=0.0, buffer_rate=0.0)
+    _, u0 = problems.minidisk(data, seed=4)
+    s = MinidiskSolver(data, precision)
+    s.set_state(u0)
+    m0 = math.fsum(u0[:, 0].ravel())
+    t = 0.0
+    for _ in range(50):
+        s.step(problems.minidisk_dt(data, t))
+        t = s.time
+    m1 = math.fsum(s.state()[:, 0].ravel())
+    assert abs(m1 - m0) / m0 <= 1e-13
+    assert abs(s.conserved_totals()[0] - m1) / m0 <= 1e-14
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_euler2d_conservation(precision):
+    from mara3p_amd import problems
+    from mara3p_amd.solvers import Euler2DSolver
+    _, u0 = problems.kelvin_helmholtz(depth=2, block_size=64, seed=8)
+    s = Euler2DSolver(2, 64, precision=precision)
+    s.set_state(u0)
+    tot0 = np.array([math.fsum(u0[:, q].ravel()) for q in range(5)])
+    for _ in range(20):
+        s.step(0.4 * s.dl / 2.0)
+    u = s.state()
+    tot = np.array([math.fsum(u[:, q].ravel()) for q in range(5)])
+    scale = np.array([math.fsum(np.abs(u0[:, q]).ravel()) for q in range(5)])
+    assert np.all(np.abs(tot - tot0) <= 1e-13 * scale)
+
+
+# --------------------------------------------------------------------------- reductions / errors
+def test_cfl_dt_matches_numpy():
+    from mara3p_amd import problems
+    from mara3p_amd.solvers import Euler2DSolver
+    _, u0 = problems.kelvin_helmholtz(depth=1, block_size=64, seed=2)
+    s = Euler2DSolver(1, 64)
+    s.set_state(u0)
+    dt = float(s.cfl_dt(0.4).item())
+    D = u0[:, 0]
+    v1, v2, v3 = u0[:, 1] / D, u0[:, 2] / D, u0[:, 3] / D
+    pg = (u0[:, 4] - 0.5 * (v1 * u0[:, 1] + v2 * u0[:, 2] + v3 * u0[:, 3])) * (5.0 / 3.0 - 1)
+    vmax = (np.maximum(np.abs(v1), np.abs(v2)) + np.sqrt(5.0 / 3.0 * pg / D)).max()
+    assert abs(dt - 0.4 * s.dl / vmax) <= 1e-14 * dt
+
+
+def test_totals_match_fsum():
+    from mara3p_amd import problems
+    from mara3p_amd.solvers import Euler2DSolver
+    _, u0 = problems.kelvin_helmholtz(depth=2, block_size=64, seed=5)
+    s = Euler2DSolver(2, 64)
+    s.set_state(u0)
+    tot = s.conserved_totals()
+    for q in range(5):
+        ref = math.fsum(u0[:, q].ravel())
+        assert abs(tot[q] - ref) <= 1e-14 * max(1.0, math.fsum(np.abs(u0[:, q]).ravel()))
+
+
+def test_negative_density_raises_reference_error():
+    from mara3p_amd import NegativeDensityError, problems
+    from mara3p_amd.solvers import Euler2DSolver
+    _, u0 = problems.kelvin_helmholtz(depth=1, block_size=64, seed=2)
+    u0 = u0.copy()
+    u0[2, 0, 10, 20] = -1.0
+    s = Euler2DSolver(1, 64)
+    s.set_state(u0)
+    s.step(1e-4)
+    with pytest.raises(NegativeDensityError):
+        s.check_errors()
+    s.check_errors()   # latch is cleared
+
+
+def test_dt_from_device_matches_host_dt():
+    from mara3p_amd import problems
+    from mara3p_amd.solvers import Euler2DSolver
+    _, u0 = problems.kelvin_helmholtz(depth=1, block_size=64, seed=6)
+    a, b = Euler2DSolver(1, 64), Euler2DSolver(1, 64)
+    a.set_state(u0)
+    b.set_state(u0)
+    dt = float(a.cfl_dt(0.4).item())
+    a.step_cfl(0.4)
+    b.step(dt)
+    assert np.array_equal(a.state(), b.state())

@@ -1,0 +1,351 @@
+"""Pin the oracle (CPU restatement) before trusting it as the checker.
+
+Each oracle function is checked against (i) golden vectors written from the
+reference's own code (tests/golden, make_golden.py), (ii) the live reference
+harness when oracle/_ref is built, (iii) the reference's own known-answer
+tests, and (iv) what the mathematics fixes (closed forms, invariants,
+symmetry), chosen so that a dropped term, wrong sign or index, or transposed
+operand fails at least one check.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import OrcMinidisk, P, orc
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (reference absent)")
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------- golden vectors
+def test_plm_golden():
+    g = golden("plm.npz")
+    got = np.array([orc().orc_plm(*y, t) for y, t in zip(g["y"], g["theta"])])
+    assert np.array_equal(got, g["g"])
+
+
+def test_euler_pointwise_golden():
+    g = golden("euler_pointwise.npz")
+    for i in range(len(g["pl"])):
+        pl, pr, ax, gm = c(g["pl"][i]), c(g["pr"][i]), int(g["axis"][i]), float(g["gamma"][i])
+        f = np.zeros(5)
+        orc().orc_euler_riemann_hlle(P(pl), P(pr), ax, gm, P(f))
+        assert np.array_equal(f, g["hlle"][i])
+        orc().orc_euler_flux(P(pl), ax, gm, P(f))
+        assert np.array_equal(f, g["flux"][i])
+        orc().orc_euler_conserved_density(P(pl), gm, P(f))
+        assert np.array_equal(f, g["cons"][i])
+        p = np.zeros(5)
+        assert orc().orc_euler_recover_primitive(P(c(g["cons"][i])), gm, P(p)) == 0
+        assert np.array_equal(p, g["recovered"][i])
+
+
+def test_iso2d_golden():
+    g = golden("iso2d_hlle.npz")
+    for i in range(len(g["pl"])):
+        f = np.zeros(3)
+        orc().orc_iso2d_riemann_hlle(P(c(g["pl"][i])), P(c(g["pr"][i])), float(g["cs2"][i]), int(g["axis"][i]), P(f))
+        assert np.array_equal(f, g["hlle"][i])
+
+
+def test_two_body_golden():
+    g = golden("two_body.npz")
+    for (q, e, t, om), st_ref in zip(g["args"], g["state"]):
+        st = np.zeros(10)
+        orc().orc_two_body_state(1.0, 1.0, q, e, t, om, P(st))
+        assert np.array_equal(st, st_ref)
+
+
+def _sd_from_array(sd):
+    return OrcMinidisk(int(sd[0]), int(sd[1]), *sd[2:])
+
+
+def test_minidisk_block_golden():
+    """minidisk::updated_conserved (problems/minidisk_scheme.cpp:294-355) on 16^2 blocks:
+    pins the shared 2D stencil skeleton (PLM face states, flux differences) and
+    the source-term composition."""
+    g = golden("minidisk_block.npz")
+    for sd, uc, pe, out, (level, bi, bj, time, dt) in zip(g["sd"], g["uc"], g["pe"], g["out"], g["meta"]):
+        s = _sd_from_array(sd)
+        got = np.zeros_like(out)
+        orc().orc_minidisk_updated_conserved(P(c(uc)), P(c(pe)), ctypes.byref(s), int(level), int(bi), int(bj),
+                                             time, dt, P(got))
+        assert np.array_equal(got, out)
+
+
+# --------------------------------------------------------------------------- live reference
+@needs_ref
+def test_minidisk_live_reference_random_blocks():
+    rng = np.random.default_rng(7)
+    for n, level in ((8, 1), (16, 3), (32, 2)):
+        for _ in range(2):
+            bi, bj = (int(x) for x in rng.integers(0, 1 << level, 2))
+            sd = np.array([n, level, 1.0, 10.0, 0.0, 1.5, 0.05, 0.1, 0.7, 0.05, 1e3, 1e2, 0.2, 0.3])
+            uc = c(np.abs(rng.normal(1, 0.1, (n, n, 3))))
+            pe = c(np.concatenate([rng.uniform(0.5, 1.5, (n + 4, n + 4, 1)), rng.normal(0, 1, (n + 4, n + 4, 2))], -1))
+            a, b = np.zeros((n, n, 3)), np.zeros((n, n, 3))
+            oracle.ref().ref_minidisk_updated_conserved(P(uc), P(pe), P(sd), level, bi, bj, 0.3, 1e-3, P(a))
+            orc().orc_minidisk_updated_conserved(P(uc), P(pe), ctypes.byref(_sd_from_array(sd)), level, bi, bj,
+                                                 0.3, 1e-3, P(b))
+            assert np.array_equal(a, b)
+
+
+@needs_ref
+def test_pointwise_live_reference():
+    rng = np.random.default_rng(11)
+    for _ in range(2000):
+        y = rng.normal(size=3)
+        th = float(rng.choice([1.0, 1.25, 2.0]))
+        assert orc().orc_plm(*y, th) == oracle.ref().ref_plm(*y, th)
+    for _ in range(500):
+        pl = c([rng.uniform(0.1, 2), *rng.normal(size=3), rng.uniform(0.1, 2)])
+        pr = c([rng.uniform(0.1, 2), *rng.normal(size=3), rng.uniform(0.1, 2)])
+        ax = int(rng.integers(1, 4))
+        a, b = np.zeros(5), np.zeros(5)
+        orc().orc_euler_riemann_hlle(P(pl), P(pr), ax, 1.4, P(a))
+        oracle.ref().ref_euler_riemann_hlle(P(pl), P(pr), ax, 1.4, P(b))
+        assert np.array_equal(a, b)
+
+
+# --------------------------------------------------------------------------- reference KATs
+@pytest.mark.parametrize("p0", [(1.0, 0, 0, 0, 1.0), (1.5, 0, 0, 0, 1e3), (1.0, 0.0, 0.0, 0.0, 1.0),
+                                (5.7, 1.0, 2.0, 3.0, 5.0)])
+def test_euler_cons_prim_round_trip(p0):
+    """test_euler, src/physics_euler.hpp:261-273 (|delta| < 1e-12 at gamma 1.4)."""
+    p0 = c(p0)
+    u, p1 = np.zeros(5), np.zeros(5)
+    orc().orc_euler_conserved_density(P(p0), 1.4, P(u))
+    assert orc().orc_euler_recover_primitive(P(u), 1.4, P(p1)) == 0
+    assert np.all(np.abs(p1 - p0) < 1e-12)
+
+
+def test_euler_negative_density_rejected():
+    u, p = c([-1.0, 0, 0, 0, 1.0]), np.zeros(5)
+    assert orc().orc_euler_recover_primitive(P(u), 1.4, P(p)) != 0
+
+
+@needs_ref
+def test_two_body_recovers_elements():
+    """test_two_body, src/physics_two_body.hpp:477-490: e = 0.5 and a = 1 recovered to 1e-10."""
+    st, el = np.zeros(10), np.zeros(4)
+    orc().orc_two_body_state(1.0, 1.0, 1.0, 0.5, 0.0, 0.0, P(st))
+    oracle.ref().ref_two_body_recover_elements(P(st), 0.0, P(el))
+    assert abs(el[3] - 0.5) < 1e-10 and abs(el[0] - 1.0) < 1e-10
+
+
+def test_two_body_circular_closed_form():
+    """e = 0, q = 1: components at radius 1/2, opposite, angular rate 1 in the inertial frame."""
+    for t in (0.0, 0.3, 2.0):
+        st = np.zeros(10)
+        orc().orc_two_body_state(1.0, 1.0, 1.0, 0.0, t, 0.0, P(st))
+        assert st[0] == 0.5 and st[5] == 0.5
+        assert np.isclose(np.hypot(st[1], st[2]), 0.5, atol=1e-15)
+        assert np.allclose(st[1:3], -st[6:8], atol=1e-15)
+        assert np.allclose(st[1:3], [0.5 * np.cos(t), 0.5 * np.sin(t)], atol=1e-14)
+        # co-rotating frame (omega_frame = 1): stationary
+        orc().orc_two_body_state(1.0, 1.0, 1.0, 0.0, t, 1.0, P(st))
+        assert np.allclose(st[1:3], [0.5, 0.0], atol=1e-14)
+
+
+# --------------------------------------------------------------------------- mathematics
+def test_plm_closed_forms():
+    f = orc().orc_plm
+    # linear data: the slope is exact for theta >= 1
+    for th in (1.0, 1.5, 2.0):
+        assert f(1.0, 2.0, 3.0, th) == 1.0
+        assert f(3.0, 1.0, -1.0, th) == -2.0
+    # extremum -> 0
+    assert f(0.0, 1.0, 0.0, 1.5) == 0.0
+    assert f(1.0, 0.0, 2.0, 2.0) == 0.0
+    # theta = 1 is minmod of one-sided differences
+    assert f(0.0, 1.0, 5.0, 1.0) == 1.0
+    # theta = 2 is monotonised central: min(2|a|, |b|, 2|c|)
+    assert f(0.0, 1.0, 5.0, 2.0) == 2.0
+    assert f(0.0, 3.0, 4.0, 2.0) == 2.0
+
+
+def _hlle(pl, pr, ax, g=1.4):
+    f = np.zeros(5)
+    orc().orc_euler_riemann_hlle(P(c(pl)), P(c(pr)), ax, g, P(f))
+    return f
+
+
+def _hllc(pl, pr, ax, g=1.4):
+    f = np.zeros(5)
+    orc().orc_euler_riemann_hllc(P(c(pl)), P(c(pr)), ax, g, P(f))
+    return f
+
+
+def _flux(p, ax, g=1.4):
+    f = np.zeros(5)
+    orc().orc_euler_flux(P(c(p)), ax, g, P(f))
+    return f
+
+
+def test_euler_flux_closed_form():
+    rho, v1, v2, v3, p, g = 1.3, 0.7, -0.2, 0.4, 0.9, 1.4
+    E = 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3) + p / (g - 1)
+    expect = [rho * v2, rho * v1 * v2, rho * v2 * v2 + p, rho * v3 * v2, (E + p) * v2]
+    assert np.allclose(_flux([rho, v1, v2, v3, p], 2, g), expect, rtol=1e-14)
+
+
+@pytest.mark.parametrize("solver", [_hlle, _hllc])
+def test_riemann_consistency_and_upwinding(solver):
+    p = [1.1, 0.3, -0.4, 0.2, 0.8]
+    for ax in (1, 2, 3):
+        assert np.allclose(solver(p, p, ax), _flux(p, ax), rtol=1e-14, atol=1e-15)
+    # supersonic to the right: the flux is the left state's flux
+    pl, pr = [1.0, 5.0, 0.1, 0, 1.0], [0.5, 4.0, -0.3, 0, 0.4]
+    assert np.array_equal(solver(pl, pr, 1), _flux(pl, 1))
+    # supersonic to the left: the right state's flux
+    pl, pr = [1.0, -5.0, 0, 0, 1.0], [0.5, -4.0, 0, 0, 0.4]
+    assert np.array_equal(solver(pl, pr, 1), _flux(pr, 1))
+
+
+@pytest.mark.parametrize("solver", [_hlle, _hllc])
+def test_riemann_mirror_symmetry(solver):
+    pl, pr = np.array([1.2, 0.3, 0.1, 0, 1.0]), np.array([0.6, -0.2, 0.4, 0, 0.5])
+    f = solver(pl, pr, 1)
+    ml, mr = pr.copy(), pl.copy()
+    ml[1] *= -1
+    mr[1] *= -1
+    fm = solver(ml, mr, 1)
+    assert np.allclose(fm[[0, 2, 4]], -f[[0, 2, 4]], rtol=1e-13)
+    assert np.allclose(fm[1], f[1], rtol=1e-13)
+
+
+def test_hllc_resolves_stationary_contact():
+    f = _hllc([2.0, 0, 0, 0, 1.0], [0.5, 0, 0, 0, 1.0], 1)
+    assert f[0] == 0.0 and f[4] == 0.0 and f[1] == 1.0
+    # HLLE diffuses it
+    assert _hlle([2.0, 0, 0, 0, 1.0], [0.5, 0, 0, 0, 1.0], 1)[0] != 0.0
+
+
+def test_iso2d_consistency():
+    p = c([1.3, 0.4, -0.7])
+    for ax in (1, 2):
+        f = np.zeros(3)
+        orc().orc_iso2d_riemann_hlle(P(p), P(p), 0.3, ax, P(f))
+        v = p[ax]
+        expect = [p[0] * v, p[0] * p[1] * v, p[0] * p[2] * v]
+        expect[ax] += p[0] * 0.3
+        assert np.allclose(f, expect, rtol=1e-14)
+
+
+def test_extend_block_2d_brute_force():
+    """amr::extend_block over a periodic level (src/mesh_amr.hpp:360-393): every guard
+    zone holds the zone of the wrapped global index."""
+    nb, n, nc = 4, 4, 2
+    N = nb * n
+    glob = np.arange(N * N * nc, dtype=np.float64).reshape(N, N, nc)
+    blocks = c(glob.reshape(nb, n, nb, n, nc).transpose(0, 2, 1, 3, 4).reshape(nb * nb, n, n, nc))
+    for bi in range(nb):
+        for bj in range(nb):
+            pe = np.zeros((n + 4, n + 4, nc))
+            orc().orc_extend_block_2d(P(blocks), nb, n, nc, bi, bj, P(pe))
+            gi = (bi * n - 2 + np.arange(n + 4)) % N
+            gj = (bj * n - 2 + np.arange(n + 4)) % N
+            assert np.array_equal(pe, glob[np.ix_(gi, gj)])
+
+
+def _kh_state(nb, n, seed=1):
+    from mara3p_amd import problems
+    from mara3p_amd.mesh import to_reference_layout
+    depth = int(np.log2(nb))
+    _, u = problems.kelvin_helmholtz(depth=depth, block_size=n, seed=seed)
+    return c(to_reference_layout(u))
+
+
+def test_euler2d_uniform_state_is_fixed_point():
+    nb, n = 2, 8
+    u = np.zeros((nb * nb, n, n, 5))
+    u[...] = [1.3, 1.3 * 0.2, 1.3 * -0.1, 0.0, 2.0]
+    u = c(u)
+    u0 = u.copy()
+    assert orc().orc_euler2d_mesh_rk2(P(u), nb, n, 1e-3, 1.0 / 16, 1.5, 1.4, 0, 3, 1) == 0
+    assert np.array_equal(u, u0)
+
+
+@pytest.mark.parametrize("hllc", [0, 1])
+def test_euler2d_periodic_conservation(hllc):
+    nb, n = 2, 16
+    u = _kh_state(nb, n)
+    tot0 = u.sum(axis=(0, 1, 2))
+    dl = 1.0 / (nb * n)
+    assert orc().orc_euler2d_mesh_rk2(P(u), nb, n, 0.2 * dl, dl, 1.5, 5.0 / 3.0, hllc, 5, 2) == 0
+    tot = u.sum(axis=(0, 1, 2))
+    assert np.all(np.abs(tot - tot0) <= 1e-12 * np.maximum(1.0, np.abs(tot0)))
+
+
+def test_euler2d_rotation_symmetry():
+    """Swapping the axes (x <-> y, v1 <-> v2) commutes with the update exactly: pins the
+    y-direction index algebra of the update skeleton against the x-direction one."""
+    nb, n = 2, 8
+    u = _kh_state(nb, n, seed=4)
+    rot = lambda a: a.reshape(nb, nb, n, n, 5).transpose(1, 0, 3, 2, 4)[..., [0, 2, 1, 3, 4]].reshape(nb * nb, n, n, 5)
+    ur = c(rot(u))
+    dl = 1.0 / (nb * n)
+    a, b = np.zeros_like(u), np.zeros_like(u)
+    orc().orc_euler2d_mesh_stage(P(u), P(a), nb, n, 0.1 * dl, dl, 1.5, 5.0 / 3.0, 0, 1)
+    orc().orc_euler2d_mesh_stage(P(ur), P(b), nb, n, 0.1 * dl, dl, 1.5, 5.0 / 3.0, 0, 1)
+    assert np.array_equal(c(rot(a)), b)
+
+
+def test_euler2d_reduces_to_euler1d():
+    """A state varying only along x: the 2D update equals the 1D module update row by row."""
+    nb, n = 2, 16
+    N = nb * n
+    x = (np.arange(N) + 0.5) / N
+    rho = 1 + 0.3 * np.sin(2 * np.pi * x)
+    prim = np.stack([rho, 0.2 + 0 * x, 0 * x, 0 * x, 1 + 0.1 * np.cos(2 * np.pi * x)], -1)
+    u1 = np.zeros((N, 5))
+    for i in range(N):
+        orc().orc_euler_conserved_density(P(c(prim[i])), 1.4, P(u1[i]))
+    g2 = np.broadcast_to(u1[:, None, :], (N, N, 5))
+    u2 = c(g2.reshape(nb, n, nb, n, 5).transpose(0, 2, 1, 3, 4).reshape(nb * nb, n, n, 5))
+    dl = 1.0 / N
+    out2 = np.zeros_like(u2)
+    orc().orc_euler2d_mesh_stage(P(u2), P(out2), nb, n, 0.1 * dl, dl, 1.5, 1.4, 0, 1)
+    # 1D: periodic chain
+    P1 = np.zeros((N, 5))
+    for i in range(N):
+        orc().orc_euler_recover_primitive(P(c(u1[i])), 1.4, P(P1[i]))
+    ext = c(P1[(np.arange(-2, N + 2)) % N])
+    out1 = np.zeros((N, 5))
+    orc().orc_euler1d_updated_conserved(P(c(u1)), P(ext), N, 0.1 * dl, dl, 1.5, 1.4, 0, 0, P(out1))
+    back = out2.reshape(nb, nb, n, n, 5).transpose(0, 2, 1, 3, 4).reshape(N, N, 5)
+    for j in (0, 7, N - 1):
+        assert np.array_equal(back[:, j, :], out1)
+
+
+def test_sod_converges_to_exact_solution():
+    """Sod (1024 zones, gamma 1.4, PLM 1.5, HLLE, RK2) to t = 0.2 against Toro's exact solution."""
+    from mara3p_amd import problems
+    nblk, n, g = 16, 64, 1.4
+    u = c(problems.sod(nblk, n, g).transpose(0, 2, 1))
+    dl = 1.0 / (nblk * n)
+    dt = 0.4 * dl / 2.2        # max |v| + c over the run is below 2.2
+    nsteps = int(np.ceil(0.2 / dt))
+    dt = 0.2 / nsteps
+    assert orc().orc_euler1d_mesh_rk2(P(u), nblk, n, dt, dl, 1.5, g, nsteps, 4) == 0
+    rho = u.reshape(-1, 5)[:, 0]
+    xc = (np.arange(nblk * n) + 0.5) * dl - 0.5
+    exact = np.zeros(3)
+    left, right = c([1.0, 0.0, 1.0]), c([0.125, 0.0, 0.1])
+    ex = np.zeros_like(xc)
+    for i, xi in enumerate(xc):
+        orc().orc_euler_exact_riemann(P(left), P(right), g, xi / 0.2, P(exact))
+        ex[i] = exact[0]
+    l1 = np.mean(np.abs(rho - ex))
+    assert l1 < 3e-3, l1
