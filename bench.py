@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--l1-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 1 GiB device copy (for HBM counter calibration under rocprofv3)")
     return ap.parse_args()
 
 
@@ -62,9 +65,10 @@ def make_solver(workload, precision):
     return s, u0, (lambda t: dt), 5
 
 
-def timed_run(s, dt_of, steps, warmup):
+def timed_run(s, dt_of, steps, warmup, autotune=True):
     """Time `steps` RK2 steps; per-launch durations from HIP events on the launch stream."""
     import torch
+    tile = s.autotune() if autotune else None
     for _ in range(warmup):
         s.step(dt_of(getattr(s, "time", 1.0)))
     torch.cuda.synchronize()
@@ -89,7 +93,7 @@ def timed_run(s, dt_of, steps, warmup):
     s.stage = orig_stage
     s.check_errors()
     durs = [a.elapsed_time(b) for a, b in ev]      # ms
-    return wall, float(np.mean(durs[0::2])), float(np.mean(durs[1::2]))
+    return wall, float(np.mean(durs[0::2])), float(np.mean(durs[1::2])), tile
 
 
 # ---------------------------------------------------------------------------
@@ -173,9 +177,15 @@ def main():
     import torch
     torch.cuda.set_device(0)
 
+    if args.copy_calibration:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     s, u0, dt_of, nc = make_solver(args.workload, "mixed")
     zones = s.mesh.zones()
-    wall, d1, d2 = timed_run(s, dt_of, args.steps, args.warmup)
+    wall, d1, d2, tile = timed_run(s, dt_of, args.steps, args.warmup, not args.no_autotune)
     ms_per_step = 1e3 * wall / args.steps
     value = zones * args.steps / wall
     # algorithmic bytes: stage 1 reads U0 writes U1 (2 nc 8 B/zone), stage 2 reads U1, U0 writes U (3 nc 8 B/zone)
@@ -200,6 +210,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles(args.workload, "mixed")},
         "stage_ms": {"stage1": d1, "stage2": d2},
+        "tile": tile,
     }
     if args.workload == "minidisk":
         result["config"] = {"workload": "minidisk iso2d+two_body, 16x16 blocks of 256^2 (16.8M zones), RK2, mixed",
@@ -212,9 +223,10 @@ def main():
     if not args.no_fp64:
         s64, _, _, _ = make_solver(args.workload, "fp64")
         k64 = max(10, args.steps // 4)
-        wall64, e1, e2 = timed_run(s64, dt_of, k64, max(2, args.warmup // 4))
+        wall64, e1, e2, tile64 = timed_run(s64, dt_of, k64, max(2, args.warmup // 4), not args.no_autotune)
         v64 = zones * k64 / wall64
         result["fp64"] = {"value": v64, "ms_per_step": 1e3 * wall64 / k64, "stage_ms": {"stage1": e1, "stage2": e2},
+                          "tile": tile64,
                           "hbm_frac": bytes_per_launch / (0.5 * (e1 + e2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
         result["mixed_over_fp64"] = value / v64
         del s64

@@ -31,6 +31,9 @@ extern "C" {
 #define M3P_ERR_HIP                5
 #define M3P_ERR_BAD_ARGUMENT       6
 
+#define M3P_KIND_EULER2D   0   /* stage2d scheme kinds, for tile selection */
+#define M3P_KIND_MINIDISK  1
+
 #define M3P_FP64   0   /* drop-in: reproduces the reference FP64 arithmetic */
 #define M3P_MIXED  1   /* FP32 reconstruction and fluxes, FP64 conserved state */
 
@@ -114,6 +117,21 @@ int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int
 int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int* nbr,
                       int nblk, int n, double dt, double dl, double rk_b,
                       double plm_theta, double gamma_law_index, int precision, void* stream);
+
+/* Tile selection and autotuning (no reference counterpart: the reference runs
+ * one closure per block on a CPU thread pool).  A 2D stage workgroup owns TJ
+ * contiguous columns of a block and marches down H rows; the compiled
+ * variants are TJ x H in {256x32, 256x16, 128x32, 64x64, 64x32, 64x16, 32x32,
+ * 16x16}.  The choice is per process, per (kind, precision). */
+int m3p_set_stage2d_tile(int kind, int precision, int tj, int h);
+int m3p_get_stage2d_tile(int kind, int precision, int* tj /* host */, int* h /* host */);
+/* Time every variant that fits on the caller's mesh (u -> out, `reps`
+ * launches each, blocking), keep the fastest and report it (host outputs). */
+int m3p_autotune_euler2d(const double* u, double* out, const int* nbr, int nblk, int n, double dl,
+                         double gamma_law_index, int precision, int reps, int* tj, int* h, float* ms, void* stream);
+int m3p_autotune_minidisk(const double* u, double* out, const int* nbr, const int* bcoord, int nblk,
+                          const m3p_minidisk_data* data /* host */, int precision, int reps,
+                          int* tj, int* h, float* ms, void* stream);
 
 /* Reductions --------------------------------------------------------------- */
 /* CFL time step for Euler states: dt = cfl * dl / max over zones of

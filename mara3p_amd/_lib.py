@@ -20,6 +20,8 @@ M3P_ERR_BAD_ARGUMENT = 6
 
 M3P_FP64 = 0
 M3P_MIXED = 1
+M3P_KIND_EULER2D = 0
+M3P_KIND_MINIDISK = 1
 
 PRECISIONS = {"fp64": M3P_FP64, "mixed": M3P_MIXED}
 
@@ -84,6 +86,12 @@ SIGNATURES = {
     "m3p_minidisk_stage": (_I, [_P, _P, _P, _P, _P, _I, _D, _D, _D, ctypes.POINTER(MinidiskData), _I, _P]),
     "m3p_euler1d_stage": (_I, [_P, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _I, _P]),
     "m3p_euler_cfl_dt": (_I, [_P, _I, _L, _D, _D, _D, _P, _P]),
+    "m3p_set_stage2d_tile": (_I, [_I, _I, _I, _I]),
+    "m3p_get_stage2d_tile": (_I, [_I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "m3p_autotune_euler2d": (_I, [_P, _P, _P, _I, _I, _D, _D, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                  ctypes.POINTER(ctypes.c_float), _P]),
+    "m3p_autotune_minidisk": (_I, [_P, _P, _P, _P, _I, ctypes.POINTER(MinidiskData), _I, _I, ctypes.POINTER(_I),
+                                   ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float), _P]),
     "m3p_conserved_totals": (_I, [_P, _I, _I, _L, _P, _P]),
 }
 
@@ -97,7 +105,7 @@ def lib():
         if not os.path.exists(LIBRARY_PATH):
             raise ImportError(
                 "mara3p_amd: HIP library %s is not built (run __graft_entry__.build() "
-                "or python -m mara3p_amd.build)" % LIBRARY_PATH)
+                "or python mara3p_amd/build.py)" % LIBRARY_PATH)
         handle = ctypes.CDLL(LIBRARY_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

@@ -1,6 +1,6 @@
 """Build recipe for the HIP library (gfx950) and the test oracle.
 
-    python -m mara3p_amd.build          # HIP library + oracle (+ reference harness when present)
+    python mara3p_amd/build.py          # HIP library + oracle (+ reference harness when present)
 
 The HIP library is compiled in-tree so the .so travels with the repository
 snapshot.  Flags:

@@ -61,26 +61,83 @@ int launch_status(const char* what)
 }
 
 // ---------------------------------------------------------------------------
-// 2D launch dispatch on block size: TJ = min(n, 256) threads per workgroup,
-// H = 32 rows per tile (16 for n = 16).
+// 2D launch dispatch.  A tile is TJ columns x H rows; the compiled variants
+// are listed below.  The variant used for each (scheme, precision) is a
+// per-process setting chosen by m3p_set_stage2d_tile or m3p_autotune_*.
+struct TileShape { int tj, h; };
+constexpr TileShape kTileVariants[] = { {256, 32}, {256, 16}, {128, 32}, {64, 64}, {64, 32}, {64, 16}, {32, 32}, {16, 16} };
+constexpr int kNumVariants = sizeof(kTileVariants) / sizeof(kTileVariants[0]);
+TileShape g_tile[2][2] = { { {64, 32}, {64, 32} }, { {64, 32}, {64, 32} } };   // [kind][precision]
+
+bool tile_fits(TileShape v, int n) { return n % v.tj == 0 && n % v.h == 0 && v.h <= n; }
+
+TileShape pick_tile(int kind, int precision, int n)
+{
+    TileShape want = g_tile[kind][precision];
+    if (tile_fits(want, n)) return want;
+    for (auto v : kTileVariants) if (tile_fits(v, n)) return v;
+    return TileShape{0, 0};
+}
+
 template<class S, class Ld>
-int launch_stage2d(const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
+int launch_stage2d_shape(TileShape ts, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
 {
     const int n = a.n;
-    auto go = [&] (auto kern, int TJ, int H) {
-        const int grid = a.nblk * (n / H) * (n / TJ);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(TJ), 0, s, prm, ld, a);
+    auto go = [&] (auto kern) {
+        const int grid = a.nblk * (n / ts.h) * (n / ts.tj);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(ts.tj), 0, s, prm, ld, a);
         return launch_status("stage2d_kernel");
     };
-    switch (n) {
-        case 16:  return go(stage2d_kernel<S, Ld, 16, 16>, 16, 16);
-        case 32:  return go(stage2d_kernel<S, Ld, 32, 32>, 32, 32);
-        case 64:  return go(stage2d_kernel<S, Ld, 32, 64>, 64, 32);
-        case 128: return go(stage2d_kernel<S, Ld, 32, 128>, 128, 32);
-        default:
-            if (n >= 256 && n % 256 == 0) return go(stage2d_kernel<S, Ld, 32, 256>, 256, 32);
+    if (ts.tj == 256 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 256>);
+    if (ts.tj == 256 && ts.h == 16) return go(stage2d_kernel<S, Ld, 16, 256>);
+    if (ts.tj == 128 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 128>);
+    if (ts.tj == 64 && ts.h == 64)  return go(stage2d_kernel<S, Ld, 64, 64>);
+    if (ts.tj == 64 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 64>);
+    if (ts.tj == 64 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 64>);
+    if (ts.tj == 32 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 32>);
+    if (ts.tj == 16 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 16>);
+    return fail(M3P_ERR_BAD_SHAPE, "m3p: 2D block size must be 16, 32 or a multiple of 64 (got " + std::to_string(n) + ")");
+}
+
+template<class S, class Ld>
+int launch_stage2d(int kind, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
+{
+    const int prec = is_fp64<typename S::Real>::value ? M3P_FP64 : M3P_MIXED;
+    return launch_stage2d_shape<S, Ld>(pick_tile(kind, prec, a.n), prm, ld, a, s);
+}
+
+// Time every tile variant that fits on the caller's arrays and keep the fastest.
+template<class S, class Ld>
+int autotune_stage2d(int kind, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s,
+                     int reps, int* best_tj, int* best_h, float* best_ms)
+{
+    const int prec = is_fp64<typename S::Real>::value ? M3P_FP64 : M3P_MIXED;
+    hipEvent_t e0, e1;
+    if (int st = hip_check(hipEventCreate(&e0), "hipEventCreate")) return st;
+    if (int st = hip_check(hipEventCreate(&e1), "hipEventCreate")) return st;
+    TileShape best{0, 0};
+    float bms = 1e30f;
+    for (auto v : kTileVariants) {
+        if (!tile_fits(v, a.n)) continue;
+        if (int st = launch_stage2d_shape<S, Ld>(v, prm, ld, a, s)) return st;       // warm-up
+        hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r)
+            if (int st = launch_stage2d_shape<S, Ld>(v, prm, ld, a, s)) return st;
+        hipEventRecord(e1, s);
+        if (int st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize")) return st;
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms /= reps;
+        if (ms < bms) { bms = ms; best = v; }
     }
-    return fail(M3P_ERR_BAD_SHAPE, "m3p: 2D block size must be 16, 32, 64, 128 or a multiple of 256 (got " + std::to_string(n) + ")");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (best.tj == 0) return fail(M3P_ERR_BAD_SHAPE, "m3p: no tile variant fits block size " + std::to_string(a.n));
+    g_tile[kind][prec] = best;
+    if (best_tj) *best_tj = best.tj;
+    if (best_h) *best_h = best.h;
+    if (best_ms) *best_ms = bms;
+    return M3P_OK;
 }
 
 template<typename R>
@@ -249,9 +306,9 @@ int m3p_euler2d_updated_conserved(const double* uc, const double* pe, double* ou
     hipStream_t s = (hipStream_t)stream;
     int st;
     if (precision == M3P_FP64)
-        st = launch_stage2d<EulerScheme<double>>(prm, ExtendedLoader<EulerScheme<double>>{uc, pe, n}, a, s);
+        st = launch_stage2d<EulerScheme<double>>(M3P_KIND_EULER2D, prm, ExtendedLoader<EulerScheme<double>>{uc, pe, n}, a, s);
     else
-        st = launch_stage2d<EulerScheme<float>>(prm, ExtendedLoader<EulerScheme<float>>{uc, pe, n}, a, s);
+        st = launch_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, ExtendedLoader<EulerScheme<float>>{uc, pe, n}, a, s);
     if (st) return st;
     return finish_blocking(stream);
 }
@@ -292,9 +349,9 @@ int m3p_minidisk_updated_conserved(const double* uc, const double* pe, double* o
     hipStream_t s = (hipStream_t)stream;
     int st;
     if (precision == M3P_FP64)
-        st = launch_stage2d<MinidiskScheme<double>>(prm, ExtendedLoader<MinidiskScheme<double>>{uc, pe, n}, a, s);
+        st = launch_stage2d<MinidiskScheme<double>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<MinidiskScheme<double>>{uc, pe, n}, a, s);
     else
-        st = launch_stage2d<MinidiskScheme<float>>(prm, ExtendedLoader<MinidiskScheme<float>>{uc, pe, n}, a, s);
+        st = launch_stage2d<MinidiskScheme<float>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<MinidiskScheme<float>>{uc, pe, n}, a, s);
     if (st) return st;
     return finish_blocking(stream);
 }
@@ -313,8 +370,8 @@ int m3p_euler2d_stage(const double* u, const double* u0, double* out, const int*
     a.n = n; a.nblk = nblk; a.err = ds->err;
     hipStream_t s = (hipStream_t)stream;
     if (precision == M3P_FP64)
-        return launch_stage2d<EulerScheme<double>>(prm, MeshLoader<EulerScheme<double>>{u, nbr, n}, a, s);
-    return launch_stage2d<EulerScheme<float>>(prm, MeshLoader<EulerScheme<float>>{u, nbr, n}, a, s);
+        return launch_stage2d<EulerScheme<double>>(M3P_KIND_EULER2D, prm, MeshLoader<EulerScheme<double>>{u, nbr, n}, a, s);
+    return launch_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, MeshLoader<EulerScheme<float>>{u, nbr, n}, a, s);
 }
 
 int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int* nbr, const int* bcoord,
@@ -334,8 +391,8 @@ int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int
     a.dl = 2.0 * data->domain_radius / double(data->block_size) / double(1 << data->depth);
     hipStream_t s = (hipStream_t)stream;
     if (precision == M3P_FP64)
-        return launch_stage2d<MinidiskScheme<double>>(prm, MeshLoader<MinidiskScheme<double>>{u, nbr, n}, a, s);
-    return launch_stage2d<MinidiskScheme<float>>(prm, MeshLoader<MinidiskScheme<float>>{u, nbr, n}, a, s);
+        return launch_stage2d<MinidiskScheme<double>>(M3P_KIND_MINIDISK, prm, MeshLoader<MinidiskScheme<double>>{u, nbr, n}, a, s);
+    return launch_stage2d<MinidiskScheme<float>>(M3P_KIND_MINIDISK, prm, MeshLoader<MinidiskScheme<float>>{u, nbr, n}, a, s);
 }
 
 int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int* nbr,
@@ -382,6 +439,55 @@ int m3p_conserved_totals(const double* u, int nblk, int nc, long zones_per_block
     hipLaunchKernelGGL(totals_partial_kernel, dim3(nparts, nc), dim3(256), 0, s, u, nblk, nc, zones_per_block, partial);
     hipLaunchKernelGGL(totals_final_kernel, dim3(nc), dim3(64), 0, s, partial, nparts, totals);
     return launch_status("m3p_conserved_totals");
+}
+
+int m3p_set_stage2d_tile(int kind, int precision, int tj, int h)
+{
+    if (kind < 0 || kind > 1 || precision < 0 || precision > 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad kind/precision");
+    for (auto v : kTileVariants)
+        if (v.tj == tj && v.h == h) { g_tile[kind][precision] = v; return M3P_OK; }
+    return fail(M3P_ERR_BAD_ARGUMENT, "m3p: tile " + std::to_string(tj) + "x" + std::to_string(h) + " is not a compiled variant");
+}
+
+int m3p_get_stage2d_tile(int kind, int precision, int* tj, int* h)
+{
+    if (kind < 0 || kind > 1 || precision < 0 || precision > 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad kind/precision");
+    if (tj) *tj = g_tile[kind][precision].tj;
+    if (h) *h = g_tile[kind][precision].h;
+    return M3P_OK;
+}
+
+int m3p_autotune_euler2d(const double* u, double* out, const int* nbr, int nblk, int n, double dl,
+                         double gamma_law_index, int precision, int reps, int* tj, int* h, float* ms, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !out || !nbr || nblk <= 0 || u == out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad autotune arguments");
+    EulerParams prm{ gamma_law_index, 1.5, 0 };
+    StageArgs a{};
+    a.out = out; a.dt = 1e-6 * dl; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = nblk; a.err = ds->err;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == M3P_FP64)
+        return autotune_stage2d<EulerScheme<double>>(M3P_KIND_EULER2D, prm, MeshLoader<EulerScheme<double>>{u, nbr, n}, a, s, reps, tj, h, ms);
+    return autotune_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, MeshLoader<EulerScheme<float>>{u, nbr, n}, a, s, reps, tj, h, ms);
+}
+
+int m3p_autotune_minidisk(const double* u, double* out, const int* nbr, const int* bcoord, int nblk,
+                          const m3p_minidisk_data* data, int precision, int reps, int* tj, int* h, float* ms, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (int st = check_minidisk_data(data)) return st;
+    if (!u || !out || !nbr || !bcoord || nblk <= 0 || u == out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad autotune arguments");
+    const int n = data->block_size;
+    MinidiskParams prm = minidisk_params(data, 0.0);
+    StageArgs a{};
+    a.out = out; a.dt = 1e-6; a.rk_b = 0.5; a.bcoord = bcoord; a.n = n; a.nblk = nblk; a.err = ds->err;
+    a.dl = 2.0 * data->domain_radius / double(data->block_size) / double(1 << data->depth);
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == M3P_FP64)
+        return autotune_stage2d<MinidiskScheme<double>>(M3P_KIND_MINIDISK, prm, MeshLoader<MinidiskScheme<double>>{u, nbr, n}, a, s, reps, tj, h, ms);
+    return autotune_stage2d<MinidiskScheme<float>>(M3P_KIND_MINIDISK, prm, MeshLoader<MinidiskScheme<float>>{u, nbr, n}, a, s, reps, tj, h, ms);
 }
 
 int m3p_locally_isothermal_updated_conserved(const double*, const double*, double*, int,

@@ -81,7 +81,12 @@ struct MinidiskScheme {
     static constexpr int NC = 3;
     using Real = R;
     using Params = MinidiskParams;
-    template<int H, int TJ> struct TileData { double xv[H + 1]; double yv[TJ + 1]; };
+    // Vertex coordinates of the tile's rows/columns in FP64 (reference formula),
+    // plus FP32 vertex and centre copies for the mixed kernel.
+    template<int H, int TJ> struct TileData {
+        double xv[H + 1]; double yv[TJ + 1];
+        float xvf[H + 1]; float xcf[H]; float yvf[TJ + 1]; float ycf[TJ];
+    };
 
     // theta = 2 is hard-coded by the reference (problems/minidisk_scheme.cpp:313-314)
     __device__ static __forceinline__ double theta(const Params&) { return 2.0; }
@@ -103,8 +108,18 @@ struct MinidiskScheme {
     template<int H, int TJ>
     __device__ static __forceinline__ void setup(TileData<H, TJ>& td, const Params& prm, const TileCtx& tc)
     {
-        for (int t = threadIdx.x; t <= H; t += blockDim.x) td.xv[t] = vertex(prm.R, tc.n, prm.level, tc.bi, tc.k0 + t);
-        for (int t = threadIdx.x; t <= TJ; t += blockDim.x) td.yv[t] = vertex(prm.R, tc.n, prm.level, tc.bj, tc.c0 + t);
+        for (int t = threadIdx.x; t <= H; t += blockDim.x) {
+            double v = vertex(prm.R, tc.n, prm.level, tc.bi, tc.k0 + t);
+            td.xv[t] = v;
+            td.xvf[t] = float(v);
+            if (t < H) td.xcf[t] = float(0.5 * (v + vertex(prm.R, tc.n, prm.level, tc.bi, tc.k0 + t + 1)));
+        }
+        for (int t = threadIdx.x; t <= TJ; t += blockDim.x) {
+            double v = vertex(prm.R, tc.n, prm.level, tc.bj, tc.c0 + t);
+            td.yv[t] = v;
+            td.yvf[t] = float(v);
+            if (t < TJ) td.ycf[t] = float(0.5 * (v + vertex(prm.R, tc.n, prm.level, tc.bj, tc.c0 + t + 1)));
+        }
     }
 
     // sound_speed_squared :127-132 at face_coordinates :134-151
@@ -112,21 +127,22 @@ struct MinidiskScheme {
     __device__ static __forceinline__ void face(const R* pl, const R* pr, const Params& prm,
                                                 const TileData<H, TJ>& td, const TileCtx& tc, int fi, int fj, R* f)
     {
-        double x, y;
-        if constexpr (AXIS == 1) {
-            x = td.xv[fi - tc.k0];
-            y = 0.5 * (td.yv[fj - tc.c0] + td.yv[fj - tc.c0 + 1]);
-        } else {
-            x = 0.5 * (td.xv[fi - tc.k0] + td.xv[fi - tc.k0 + 1]);
-            y = td.yv[fj - tc.c0];
-        }
         R cs2;
         if constexpr (is_fp64<R>::value) {
+            double x, y;
+            if constexpr (AXIS == 1) {
+                x = td.xv[fi - tc.k0];
+                y = 0.5 * (td.yv[fj - tc.c0] + td.yv[fj - tc.c0 + 1]);
+            } else {
+                x = 0.5 * (td.xv[fi - tc.k0] + td.xv[fi - tc.k0 + 1]);
+                y = td.yv[fj - tc.c0];
+            }
             double r2 = x * x + y * y;
             double ph = -1.0 * 1.0 / sqrt(r2 + prm.s2);
             cs2 = -ph / prm.mach / prm.mach;
         } else {
-            float xf = float(x), yf = float(y);
+            float xf = AXIS == 1 ? td.xvf[fi - tc.k0] : td.xcf[fi - tc.k0];
+            float yf = AXIS == 1 ? td.ycf[fj - tc.c0] : td.yvf[fj - tc.c0];
             float q = fmaf(xf, xf, fmaf(yf, yf, float(prm.s2)));
             cs2 = __builtin_amdgcn_rsqf(q) * float(1.0 / (prm.mach * prm.mach));
         }
@@ -140,9 +156,9 @@ struct MinidiskScheme {
                                                   const double* dfx, const double* dfy,
                                                   double dt, double dl, double* out)
     {
-        double x = 0.5 * (td.xv[k - tc.k0] + td.xv[k - tc.k0 + 1]);
-        double y = 0.5 * (td.yv[c - tc.c0] + td.yv[c - tc.c0 + 1]);
         if constexpr (is_fp64<R>::value) {
+            double x = 0.5 * (td.xv[k - tc.k0] + td.xv[k - tc.k0 + 1]);
+            double y = 0.5 * (td.yv[c - tc.c0] + td.yv[c - tc.c0 + 1]);
             // two_body::gravitational_acceleration (src/physics_two_body.hpp:208-218)
             double dx1 = x - prm.x1, dy1 = y - prm.y1;
             double dx2 = x - prm.x2, dy2 = y - prm.y2;
@@ -184,7 +200,7 @@ struct MinidiskScheme {
                 out[q] = u[q] - (dfx[q] + dfy[q]) * dt / dl + src * dt;
             }
         } else {
-            float xf = float(x), yf = float(y);
+            float xf = td.xcf[k - tc.k0], yf = td.ycf[c - tc.c0];
             float s2 = float(prm.s2);
             float dx1 = xf - float(prm.x1), dy1 = yf - float(prm.y1);
             float dx2 = xf - float(prm.x2), dy2 = yf - float(prm.y2);

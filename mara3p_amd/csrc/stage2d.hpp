@@ -28,6 +28,8 @@ struct MeshLoader {
     const int* nbr;
     int n;
 
+    // Pointer to zone (r, c) of block b; rows and columns one block outside
+    // [0, n) are taken from the neighbouring block.
     __device__ __forceinline__ const double* zone(int b, int r, int c) const
     {
         if (r < 0)        { b = nbr[4 * b + 0]; r += n; }
@@ -36,18 +38,34 @@ struct MeshLoader {
         else if (c >= n)  { b = nbr[4 * b + 3]; c -= n; }
         return u + ((size_t)b * S::NC * n + r) * n + c;
     }
-    __device__ __forceinline__ void cons(int b, int r, int c, double* uu) const
+    // Start of row r of block b (wave-uniform); columns must lie in [0, n).
+    __device__ __forceinline__ const double* row(int b, int r) const
     {
-        const double* z = zone(b, r, c);
+        if (r < 0)        { b = nbr[4 * b + 0]; r += n; }
+        else if (r >= n)  { b = nbr[4 * b + 1]; r -= n; }
+        return u + ((size_t)b * S::NC * n + r) * n;
+    }
+    __device__ __forceinline__ void cons_at(const double* z, double* uu) const
+    {
 #pragma unroll
         for (int q = 0; q < S::NC; ++q) uu[q] = z[(size_t)q * n * n];
     }
+    __device__ __forceinline__ void cons(int b, int r, int c, double* uu) const { cons_at(zone(b, r, c), uu); }
     // Returns false when the recovery flagged the zone (negative density).
     __device__ __forceinline__ bool prim(int b, int r, int c, const typename S::Params& prm,
                                          typename S::Real* p) const
     {
         double uu[S::NC];
         cons(b, r, c, uu);
+        return S::recover(uu, prm, p);
+    }
+    // Interior-column variants used by the march (c in [0, n)).
+    __device__ __forceinline__ void cons_row(int b, int r, int c, double* uu) const { cons_at(row(b, r) + c, uu); }
+    __device__ __forceinline__ bool prim_row(int b, int r, int c, const typename S::Params& prm,
+                                             typename S::Real* p) const
+    {
+        double uu[S::NC];
+        cons_at(row(b, r) + c, uu);
         return S::recover(uu, prm, p);
     }
 };
@@ -72,6 +90,12 @@ struct ExtendedLoader {
 #pragma unroll
         for (int q = 0; q < S::NC; ++q) p[q] = typename S::Real(pe[((size_t)q * e + (r + 2)) * e + (c + 2)]);
         return true;
+    }
+    __device__ __forceinline__ void cons_row(int b, int r, int c, double* uu) const { cons(b, r, c, uu); }
+    __device__ __forceinline__ bool prim_row(int b, int r, int c, const typename S::Params& prm,
+                                             typename S::Real* p) const
+    {
+        return prim(b, r, c, prm, p);
     }
 };
 
@@ -155,10 +179,10 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
     R Pa[NC], Pb[NC], ga[NC], Fxp[NC];
     {
         R Pm2[NC], Pm1[NC], gm[NC], pl[NC], pr[NC];
-        ok &= ld.prim(b, tc.k0 - 2, c, prm, Pm2);
-        ok &= ld.prim(b, tc.k0 - 1, c, prm, Pm1);
-        ok &= ld.prim(b, tc.k0, c, prm, Pa);
-        ok &= ld.prim(b, tc.k0 + 1, c, prm, Pb);
+        ok &= ld.prim_row(b, tc.k0 - 2, c, prm, Pm2);
+        ok &= ld.prim_row(b, tc.k0 - 1, c, prm, Pm1);
+        ok &= ld.prim_row(b, tc.k0, c, prm, Pa);
+        ok &= ld.prim_row(b, tc.k0 + 1, c, prm, Pb);
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
             gm[m] = plm(Pm2[m], Pm1[m], Pa[m], theta);
@@ -174,7 +198,7 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
     for (int kk = 0; kk < H; ++kk) {
         const int k = tc.k0 + kk;
         R Pc[NC], gb[NC], pl[NC], pr[NC], Fxn[NC], gy[NC], Fy[NC];
-        ok &= ld.prim(b, k + 2, c, prm, Pc);
+        ok &= ld.prim_row(b, k + 2, c, prm, Pc);
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
             gb[m] = plm(Pa[m], Pb[m], Pc[m], theta);
@@ -216,7 +240,7 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
             dfx[m] = double(Fxn[m]) - double(Fxp[m]);
             dfy[m] = double(rowF[m][t + 1]) - double(Fy[m]);
         }
-        ld.cons(b, k, c, uu);
+        ld.cons_row(b, k, c, uu);
         S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, a.dl, res);
         double* o = a.out + ((size_t)b * NC * n + k) * n + c;
         if (a.u0) {

@@ -80,6 +80,14 @@ class Euler2DSolver(_StateSolver):
                                      _ptr(self.dt_dev), _stream()))
         return self.dt_dev
 
+    def autotune(self, reps=5):
+        """Pick the fastest compiled tile shape for this mesh (blocking); returns (tj, h, ms)."""
+        tj, h, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0)
+        check(lib().m3p_autotune_euler2d(_ptr(self.u), _ptr(self.u1), _ptr(self.nbr), self.mesh.nblk, self.mesh.n,
+                                         self.dl, self.gamma, self.precision, reps, ctypes.byref(tj), ctypes.byref(h),
+                                         ctypes.byref(ms), _stream()))
+        return tj.value, h.value, ms.value
+
     def step_cfl(self, cfl):
         """One RK2 step with dt = cfl dl / max(|v| + c_s) kept on the device."""
         dt_dev = self.cfl_dt(cfl)
@@ -106,6 +114,14 @@ class MinidiskSolver(_StateSolver):
         check(lib().m3p_minidisk_stage(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), _ptr(self.bcoord),
                                        self.mesh.nblk, time, dt, rk_b, ctypes.byref(self.data),
                                        self.precision, _stream()))
+
+    def autotune(self, reps=5):
+        """Pick the fastest compiled tile shape for this mesh (blocking); returns (tj, h, ms)."""
+        tj, h, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0)
+        check(lib().m3p_autotune_minidisk(_ptr(self.u), _ptr(self.u1), _ptr(self.nbr), _ptr(self.bcoord),
+                                          self.mesh.nblk, ctypes.byref(self.data), self.precision, reps,
+                                          ctypes.byref(tj), ctypes.byref(h), ctypes.byref(ms), _stream()))
+        return tj.value, h.value, ms.value
 
     def step(self, dt):
         t = self.time

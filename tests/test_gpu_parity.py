@@ -79,8 +79,9 @@ def test_euler2d_updated_conserved_fp64_exact(n, hllc):
     ref = np.zeros_like(uc)
     orc().orc_euler2d_updated_conserved(P(uc), P(pe), n, n, dt, dl, 1.5, 1.4, hllc, P(ref))
     out = torch.empty((5, n, n), dtype=torch.float64, device="cuda")
-    m3p.check(m3p.lib().m3p_euler2d_updated_conserved(ptr(dev(np.moveaxis(uc, -1, 0))), ptr(dev(np.moveaxis(pe, -1, 0))),
-                                                      ptr(out), n, dt, dl, 1.5, 1.4, hllc, m3p.M3P_FP64, stream()))
+    d_uc, d_pe = dev(np.moveaxis(uc, -1, 0)), dev(np.moveaxis(pe, -1, 0))   # held until the kernel has run
+    m3p.check(m3p.lib().m3p_euler2d_updated_conserved(ptr(d_uc), ptr(d_pe), ptr(out), n, dt, dl, 1.5, 1.4, hllc,
+                                                      m3p.M3P_FP64, stream()))
     got = np.moveaxis(out.cpu().numpy(), 0, -1)
     assert np.array_equal(got, ref)
 
@@ -99,7 +100,8 @@ def test_euler1d_updated_conserved_fp64_exact():
         ref = np.zeros_like(uc)
         orc().orc_euler1d_updated_conserved(P(uc), P(pe), n, 1e-3, 1.0 / n, 1.5, 1.4, lo, hi, P(ref))
         out = torch.empty((5, n), dtype=torch.float64, device="cuda")
-        m3p.check(m3p.lib().m3p_euler1d_updated_conserved(ptr(dev(uc.T)), ptr(dev(pe.T)), ptr(out), n, 1e-3, 1.0 / n,
+        d_uc, d_pe = dev(uc.T), dev(pe.T)
+        m3p.check(m3p.lib().m3p_euler1d_updated_conserved(ptr(d_uc), ptr(d_pe), ptr(out), n, 1e-3, 1.0 / n,
                                                           1.5, 1.4, lo, hi, m3p.M3P_FP64, stream()))
         assert np.array_equal(out.cpu().numpy().T, ref)
 
@@ -121,9 +123,9 @@ def test_minidisk_updated_conserved_fp64(n, level, bi, bj):
     ref = np.zeros_like(uc)
     orc().orc_minidisk_updated_conserved(P(uc), P(pe), ctypes.byref(sd), level, bi, bj, 0.9, 1e-3, P(ref))
     out = torch.empty((3, n, n), dtype=torch.float64, device="cuda")
-    m3p.check(m3p.lib().m3p_minidisk_updated_conserved(ptr(dev(np.moveaxis(uc, -1, 0))), ptr(dev(np.moveaxis(pe, -1, 0))),
-                                                       ptr(out), level, bi, bj, 0.9, 1e-3, ctypes.byref(data),
-                                                       m3p.M3P_FP64, stream()))
+    d_uc, d_pe = dev(np.moveaxis(uc, -1, 0)), dev(np.moveaxis(pe, -1, 0))
+    m3p.check(m3p.lib().m3p_minidisk_updated_conserved(ptr(d_uc), ptr(d_pe), ptr(out), level, bi, bj, 0.9, 1e-3,
+                                                       ctypes.byref(data), m3p.M3P_FP64, stream()))
     got = np.moveaxis(out.cpu().numpy(), 0, -1)
     err = rel_err(got[None].transpose(0, 3, 1, 2), ref[None].transpose(0, 3, 1, 2))
     assert err <= 1e-12, err
