@@ -93,6 +93,19 @@ int m3p_locally_isothermal_updated_conserved(const double* uc, const double* pe,
                                    double dt, double dl, const double* cs2_x, const double* cs2_y,
                                    int precision, void* stream);
 
+/* Block helpers of the module API --------------------------------------- */
+/* recover_primitive_array, src/module_euler.cpp:84-89 (euler1d) and :240-245
+ * (euler2d): u, p [5][nzones]; blocking; negative density reported like the
+ * reference's throw (src/physics_euler.hpp:158-161). */
+int m3p_euler_recover_primitive_array(const double* u, double* p, long nzones, double gamma_law_index, void* stream);
+/* recover_primitive_array, src/module_locally_isothermal.cpp:86-91: [3][nzones]. */
+int m3p_iso2d_recover_primitive_array(const double* u, double* p, long nzones, void* stream);
+/* estimate_gradient, src/module_euler.cpp:251-257: PLM slopes of p
+ * [nc][ni][nj] along `axis` (0 or 1) into g [nc][ni-2][nj] or [nc][ni][nj-2]. */
+int m3p_plm_gradient_2d(const double* p, double* g, int nc, int ni, int nj, int axis, double plm_theta, void* stream);
+/* weighted_sum / weighted_sum_tree, src/mesh_amr.hpp:455-467: out = s*b + t*(1-b). */
+int m3p_weighted_sum(const double* s, const double* t, double* out, long n, double b, void* stream);
+
 /* Whole-mesh stage entry points (throughput path) ------------------------- */
 /* One RK stage over every block of a mesh, guard fill fused: zones outside a
  * block are read from its neighbours through nbr [nblk][4] (device) =

@@ -93,6 +93,10 @@ SIGNATURES = {
     "m3p_autotune_minidisk": (_I, [_P, _P, _P, _P, _I, ctypes.POINTER(MinidiskData), _I, _I, ctypes.POINTER(_I),
                                    ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float), _P]),
     "m3p_conserved_totals": (_I, [_P, _I, _I, _L, _P, _P]),
+    "m3p_euler_recover_primitive_array": (_I, [_P, _P, _L, _D, _P]),
+    "m3p_iso2d_recover_primitive_array": (_I, [_P, _P, _L, _P]),
+    "m3p_plm_gradient_2d": (_I, [_P, _P, _I, _I, _I, _I, _D, _P]),
+    "m3p_weighted_sum": (_I, [_P, _P, _P, _L, _D, _P]),
 }
 
 _lib = None

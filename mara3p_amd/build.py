@@ -55,10 +55,22 @@ def build_oracle(verbose=False):
         _run(["make", "-s", "-C", oracle, "REFERENCE=" + REFERENCE, "ref"], verbose)
 
 
+def build_host_tests(verbose=False):
+    """C++ test of the host API against the oracle (tests/cpp)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "test_host_api")
+    _run([HIPCC, "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(PKG, "host"),
+          src, "-o", out + ".tmp",
+          "-L" + PKG, "-lmara3p_amd", "-L" + os.path.join(ROOT, "oracle"), "-loracle",
+          "-Wl,-rpath,$ORIGIN/../../mara3p_amd:$ORIGIN/../../oracle"], verbose)
+    os.replace(out + ".tmp", out)
+
+
 def build(verbose=False):
     build_hip(verbose)
     if shutil.which("make") and shutil.which("gcc"):
         build_oracle(verbose)
+        build_host_tests(verbose)
 
 
 if __name__ == "__main__":

@@ -6,6 +6,7 @@
 #include <mutex>
 #include <string>
 #include "mara3p_amd.h"
+#include "elementwise.hpp"
 #include "stage1d.hpp"
 #include "stage2d.hpp"
 #include "two_body.hpp"
@@ -67,7 +68,7 @@ int launch_status(const char* what)
 struct TileShape { int tj, h; };
 constexpr TileShape kTileVariants[] = { {256, 32}, {256, 16}, {128, 32}, {64, 64}, {64, 32}, {64, 16}, {32, 32}, {16, 16} };
 constexpr int kNumVariants = sizeof(kTileVariants) / sizeof(kTileVariants[0]);
-TileShape g_tile[2][2] = { { {64, 32}, {64, 32} }, { {64, 32}, {64, 32} } };   // [kind][precision]
+TileShape g_tile[2][2] = { { {256, 16}, {256, 32} }, { {256, 16}, {256, 16} } };   // [kind][precision]
 
 bool tile_fits(TileShape v, int n) { return n % v.tj == 0 && n % v.h == 0 && v.h <= n; }
 
@@ -82,20 +83,23 @@ TileShape pick_tile(int kind, int precision, int n)
 template<class S, class Ld>
 int launch_stage2d_shape(TileShape ts, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
 {
+    // The mixed kernels keep the conserved state in registers between its load
+    // and its update; the FP64 kernels (register-bound) re-load it.
+    constexpr bool K = !is_fp64<typename S::Real>::value;
     const int n = a.n;
     auto go = [&] (auto kern) {
         const int grid = a.nblk * (n / ts.h) * (n / ts.tj);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(ts.tj), 0, s, prm, ld, a);
         return launch_status("stage2d_kernel");
     };
-    if (ts.tj == 256 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 256>);
-    if (ts.tj == 256 && ts.h == 16) return go(stage2d_kernel<S, Ld, 16, 256>);
-    if (ts.tj == 128 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 128>);
-    if (ts.tj == 64 && ts.h == 64)  return go(stage2d_kernel<S, Ld, 64, 64>);
-    if (ts.tj == 64 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 64>);
-    if (ts.tj == 64 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 64>);
-    if (ts.tj == 32 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 32>);
-    if (ts.tj == 16 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 16>);
+    if (ts.tj == 256 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 256, K>);
+    if (ts.tj == 256 && ts.h == 16) return go(stage2d_kernel<S, Ld, 16, 256, K>);
+    if (ts.tj == 128 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 128, K>);
+    if (ts.tj == 64 && ts.h == 64)  return go(stage2d_kernel<S, Ld, 64, 64, K>);
+    if (ts.tj == 64 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 64, K>);
+    if (ts.tj == 64 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 64, K>);
+    if (ts.tj == 32 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 32, K>);
+    if (ts.tj == 16 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 16, K>);
     return fail(M3P_ERR_BAD_SHAPE, "m3p: 2D block size must be 16, 32 or a multiple of 64 (got " + std::to_string(n) + ")");
 }
 
@@ -441,6 +445,47 @@ int m3p_conserved_totals(const double* u, int nblk, int nc, long zones_per_block
     return launch_status("m3p_conserved_totals");
 }
 
+int m3p_euler_recover_primitive_array(const double* u, double* p, long nzones, double gamma_law_index, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !p || nzones <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad recover arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(euler_recover_array_kernel, dim3((nzones + 255) / 256), dim3(256), 0, s, u, p, nzones, gamma_law_index, ds->err);
+    if (int st = launch_status("euler_recover_array_kernel")) return st;
+    return finish_blocking(stream);
+}
+
+int m3p_iso2d_recover_primitive_array(const double* u, double* p, long nzones, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !p || nzones <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad recover arguments");
+    hipLaunchKernelGGL(iso2d_recover_array_kernel, dim3((nzones + 255) / 256), dim3(256), 0, (hipStream_t)stream, u, p, nzones);
+    return launch_status("iso2d_recover_array_kernel");
+}
+
+int m3p_plm_gradient_2d(const double* p, double* g, int nc, int ni, int nj, int axis, double plm_theta, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!p || !g || nc <= 0 || axis < 0 || axis > 1 || (axis == 0 ? ni : nj) < 3 || ni <= 0 || nj <= 0)
+        return fail(M3P_ERR_BAD_SHAPE, "m3p: bad gradient shape");
+    long nout = (long)(axis == 0 ? ni - 2 : ni) * (axis == 1 ? nj - 2 : nj) * nc;
+    hipLaunchKernelGGL(plm_gradient_2d_kernel, dim3((nout + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       p, g, nc, ni, nj, axis, plm_theta);
+    return launch_status("plm_gradient_2d_kernel");
+}
+
+int m3p_weighted_sum(const double* s, const double* t, double* out, long n, double b, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!s || !t || !out || n <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad weighted_sum arguments");
+    hipLaunchKernelGGL(weighted_sum_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, s, t, out, n, b);
+    return launch_status("weighted_sum_kernel");
+}
+
 int m3p_set_stage2d_tile(int kind, int precision, int tj, int h)
 {
     if (kind < 0 || kind > 1 || precision < 0 || precision > 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad kind/precision");
@@ -490,10 +535,24 @@ int m3p_autotune_minidisk(const double* u, double* out, const int* nbr, const in
     return autotune_stage2d<MinidiskScheme<float>>(M3P_KIND_MINIDISK, prm, MeshLoader<MinidiskScheme<float>>{u, nbr, n}, a, s, reps, tj, h, ms);
 }
 
-int m3p_locally_isothermal_updated_conserved(const double*, const double*, double*, int,
-                                   double, double, const double*, const double*, int, void*)
+int m3p_locally_isothermal_updated_conserved(const double* uc, const double* pe, double* out, int n,
+                                   double dt, double dl, const double* cs2_x, const double* cs2_y,
+                                   int precision, void* stream)
 {
-    return fail(M3P_ERR_BAD_ARGUMENT, "m3p_locally_isothermal_updated_conserved: not built yet");
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!uc || !pe || !out || !cs2_x || !cs2_y) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: NULL array");
+    IsoTableParams prm{ cs2_x, cs2_y, n };
+    StageArgs a{};
+    a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = ds->err;
+    hipStream_t s = (hipStream_t)stream;
+    int st;
+    if (precision == M3P_FP64)
+        st = launch_stage2d<IsoTableScheme<double>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<IsoTableScheme<double>>{uc, pe, n}, a, s);
+    else
+        st = launch_stage2d<IsoTableScheme<float>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<IsoTableScheme<float>>{uc, pe, n}, a, s);
+    if (st) return st;
+    return finish_blocking(stream);
 }
 
 } // extern "C"

@@ -233,4 +233,52 @@ struct MinidiskScheme {
     }
 };
 
+// ---------------------------------------------------------------------------
+// locally_isothermal::updated_conserved with zero viscosity: the face sound
+// speeds squared come from arrays cs2x [(n+1)][n] and cs2y [n][(n+1)]
+// evaluated by the caller at the face coordinates (the reference evaluates a
+// host std::function per face, src/module_locally_isothermal.cpp:115-121).
+struct IsoTableParams {
+    const double* cs2x;
+    const double* cs2y;
+    int n;
+};
+
+template<typename R>
+struct IsoTableScheme {
+    static constexpr int NC = 3;
+    using Real = R;
+    using Params = IsoTableParams;
+    template<int H, int TJ> struct TileData {};
+
+    // theta = 2 is hard-coded by the reference (src/module_locally_isothermal.cpp:195-196)
+    __device__ static __forceinline__ double theta(const Params&) { return 2.0; }
+    __device__ static __forceinline__ bool recover(const double* u, const Params&, R* p)
+    {
+        Iso2d<R>::recover(u, p);
+        return true;
+    }
+    template<int H, int TJ>
+    __device__ static __forceinline__ void setup(TileData<H, TJ>&, const Params&, const TileCtx&) {}
+
+    template<int AXIS, int H, int TJ>
+    __device__ static __forceinline__ void face(const R* pl, const R* pr, const Params& prm,
+                                                const TileData<H, TJ>&, const TileCtx&, int fi, int fj, R* f)
+    {
+        R cs2 = R(AXIS == 1 ? prm.cs2x[(size_t)fi * prm.n + fj] : prm.cs2y[(size_t)fi * (prm.n + 1) + fj]);
+        Iso2d<R>::template hlle<AXIS>(pl, pr, cs2, f);
+    }
+
+    // U - ((dFx + dFy) * dt) / dl   (src/module_locally_isothermal.cpp:209)
+    template<int H, int TJ>
+    __device__ static __forceinline__ void update(const Params&, const TileData<H, TJ>&, const TileCtx&,
+                                                  int, int, const double* u, const R*,
+                                                  const double* dfx, const double* dfy,
+                                                  double dt, double dl, double* out)
+    {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) out[q] = u[q] - (dfx[q] + dfy[q]) * dt / dl;
+    }
+};
+
 } // namespace m3p

@@ -61,6 +61,13 @@ struct MeshLoader {
     }
     // Interior-column variants used by the march (c in [0, n)).
     __device__ __forceinline__ void cons_row(int b, int r, int c, double* uu) const { cons_at(row(b, r) + c, uu); }
+    // Conserved state and primitives of zone (r, c) in one load.
+    __device__ __forceinline__ bool load_row(int b, int r, int c, const typename S::Params& prm,
+                                             double* uu, typename S::Real* p) const
+    {
+        cons_at(row(b, r) + c, uu);
+        return S::recover(uu, prm, p);
+    }
     __device__ __forceinline__ bool prim_row(int b, int r, int c, const typename S::Params& prm,
                                              typename S::Real* p) const
     {
@@ -97,6 +104,15 @@ struct ExtendedLoader {
     {
         return prim(b, r, c, prm, p);
     }
+    // Primitives from pe; the conserved state from uc for interior rows only
+    // (guard rows are never updated).
+    __device__ __forceinline__ bool load_row(int b, int r, int c, const typename S::Params& prm,
+                                             double* uu, typename S::Real* p) const
+    {
+        if (r >= 0 && r < n) cons(b, r, c, uu);
+        else for (int q = 0; q < S::NC; ++q) uu[q] = 0.0;
+        return prim(b, r, c, prm, p);
+    }
 };
 
 struct StageArgs {
@@ -113,7 +129,10 @@ struct StageArgs {
     int* err;
 };
 
-template<class S, class Ld, int H, int TJ>
+// KEEPU: keep each zone's conserved state in a three-row register window from
+// its load to its update (no second read of the state); otherwise re-load it
+// at the update (fewer registers).
+template<class S, class Ld, int H, int TJ, bool KEEPU>
 __global__ void __launch_bounds__(TJ)
 stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
 {
@@ -177,12 +196,18 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
 
     // ---- x warm-up: rows k0-2 .. k0+1, flux at face k0 ------------------
     R Pa[NC], Pb[NC], ga[NC], Fxp[NC];
+    double Ua[NC], Ub[NC];
     {
         R Pm2[NC], Pm1[NC], gm[NC], pl[NC], pr[NC];
         ok &= ld.prim_row(b, tc.k0 - 2, c, prm, Pm2);
         ok &= ld.prim_row(b, tc.k0 - 1, c, prm, Pm1);
-        ok &= ld.prim_row(b, tc.k0, c, prm, Pa);
-        ok &= ld.prim_row(b, tc.k0 + 1, c, prm, Pb);
+        if constexpr (KEEPU) {
+            ok &= ld.load_row(b, tc.k0, c, prm, Ua, Pa);
+            ok &= ld.load_row(b, tc.k0 + 1, c, prm, Ub, Pb);
+        } else {
+            ok &= ld.prim_row(b, tc.k0, c, prm, Pa);
+            ok &= ld.prim_row(b, tc.k0 + 1, c, prm, Pb);
+        }
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
             gm[m] = plm(Pm2[m], Pm1[m], Pa[m], theta);
@@ -198,7 +223,9 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
     for (int kk = 0; kk < H; ++kk) {
         const int k = tc.k0 + kk;
         R Pc[NC], gb[NC], pl[NC], pr[NC], Fxn[NC], gy[NC], Fy[NC];
-        ok &= ld.prim_row(b, k + 2, c, prm, Pc);
+        double Uc[NC];
+        if constexpr (KEEPU) ok &= ld.load_row(b, k + 2, c, prm, Uc, Pc);
+        else                 ok &= ld.prim_row(b, k + 2, c, prm, Pc);
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
             gb[m] = plm(Pa[m], Pb[m], Pc[m], theta);
@@ -240,7 +267,12 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
             dfx[m] = double(Fxn[m]) - double(Fxp[m]);
             dfy[m] = double(rowF[m][t + 1]) - double(Fy[m]);
         }
-        ld.cons_row(b, k, c, uu);
+        if constexpr (KEEPU) {
+#pragma unroll
+            for (int m = 0; m < NC; ++m) uu[m] = Ua[m];
+        } else {
+            ld.cons_row(b, k, c, uu);
+        }
         S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, a.dl, res);
         double* o = a.out + ((size_t)b * NC * n + k) * n + c;
         if (a.u0) {
@@ -253,6 +285,10 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
         }
 #pragma unroll
         for (int m = 0; m < NC; ++m) { Pa[m] = Pb[m]; Pb[m] = Pc[m]; ga[m] = gb[m]; Fxp[m] = Fxn[m]; }
+        if constexpr (KEEPU) {
+#pragma unroll
+            for (int m = 0; m < NC; ++m) { Ua[m] = Ub[m]; Ub[m] = Uc[m]; }
+        }
     }
     if (!ok) report_error(a.err, 1, b, 0);
 }
