@@ -131,6 +131,29 @@ int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int*
                       int nblk, int n, double dt, double dl, double rk_b,
                       double plm_theta, double gamma_law_index, int precision, void* stream);
 
+/* Ideal MHD with constrained transport ------------------------------------ */
+/* mara::advance_mhd_ct, src/scheme_mhd_v1.cpp:265-299 (declared
+ * src/scheme_mhd_v1.hpp:108-118), periodic cube of N^3 cells, PLM + HLLE,
+ * edge EMFs averaged from face EMFs, dt = cfl * dl / max fastest speed
+ * computed by this call (written to dt, device double, may be NULL).
+ *   uc, uo [5][N][N][N]; bf1, bo1 [N+1][N][N]; bf2, bo2 [N][N+1][N];
+ *   bf3, bo3 [N][N][N+1]   (the reference's face arrays, both periodic end
+ *   faces stored).
+ * With u0 != NULL the RK combine out = base * rk_b + advance * (1 - rk_b) is
+ * fused (examples/mhd3d.cpp:77-86 weighted_sum).  work: device scratch of at
+ * least m3p_mhd_workspace_bytes(N, precision) bytes (primitives, face fluxes
+ * and face fields).  Asynchronous; negative density / pressure are latched
+ * (M3P_ERR_NEGATIVE_DENSITY / _PRESSURE, src/physics_mhd.hpp:296-300). */
+long m3p_mhd_workspace_bytes(int N, int precision);
+int  m3p_mhd_advance_ct(const double* uc, const double* bf1, const double* bf2, const double* bf3,
+                        const double* u0, const double* b01, const double* b02, const double* b03, double rk_b,
+                        int N, double dl, double cfl_number, double plm_theta, double gamma_law_index,
+                        double* uo, double* bo1, double* bo2, double* bo3, double* dt,
+                        void* work, long work_bytes, int precision, void* stream);
+/* max over cells of |div B| = |d1 B1 + d2 B2 + d3 B3| (divergence_difference,
+ * src/mesh_cartesian_3d.hpp:290-301) into out (device double). */
+int  m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, int N, double* out, void* stream);
+
 /* Tile selection and autotuning (no reference counterpart: the reference runs
  * one closure per block on a CPU thread pool).  A 2D stage workgroup owns TJ
  * contiguous columns of a block and marches down H rows; the compiled

@@ -97,6 +97,10 @@ SIGNATURES = {
     "m3p_iso2d_recover_primitive_array": (_I, [_P, _P, _L, _P]),
     "m3p_plm_gradient_2d": (_I, [_P, _P, _I, _I, _I, _I, _D, _P]),
     "m3p_weighted_sum": (_I, [_P, _P, _P, _L, _D, _P]),
+    "m3p_mhd_workspace_bytes": (_L, [_I, _I]),
+    "m3p_mhd_advance_ct": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P,
+                                _P, _L, _I, _P]),
+    "m3p_mhd_max_div_b": (_I, [_P, _P, _P, _I, _P, _P]),
 }
 
 _lib = None

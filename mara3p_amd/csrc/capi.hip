@@ -7,6 +7,7 @@
 #include <string>
 #include "mara3p_amd.h"
 #include "elementwise.hpp"
+#include "mhd.hpp"
 #include "stage1d.hpp"
 #include "stage2d.hpp"
 #include "two_body.hpp"
@@ -124,18 +125,18 @@ int autotune_stage2d(int kind, const typename S::Params& prm, const Ld& ld, cons
     for (auto v : kTileVariants) {
         if (!tile_fits(v, a.n)) continue;
         if (int st = launch_stage2d_shape<S, Ld>(v, prm, ld, a, s)) return st;       // warm-up
-        hipEventRecord(e0, s);
+        (void)hipEventRecord(e0, s);
         for (int r = 0; r < reps; ++r)
             if (int st = launch_stage2d_shape<S, Ld>(v, prm, ld, a, s)) return st;
-        hipEventRecord(e1, s);
+        (void)hipEventRecord(e1, s);
         if (int st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize")) return st;
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
         ms /= reps;
         if (ms < bms) { bms = ms; best = v; }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (best.tj == 0) return fail(M3P_ERR_BAD_SHAPE, "m3p: no tile variant fits block size " + std::to_string(a.n));
     g_tile[kind][prec] = best;
     if (best_tj) *best_tj = best.tj;
@@ -222,6 +223,11 @@ euler_max_speed_kernel(const double* u, int nblk, long zpb, EulerParams prm, uns
     }
 }
 
+__global__ void copy_u64_as_double_kernel(const unsigned long long* slot, double* out)
+{
+    *out = __longlong_as_double((long long)*slot);
+}
+
 __global__ void cfl_finish_kernel(const unsigned long long* slot, double cfl, double dl, double* dt_out)
 {
     double vmax = __longlong_as_double((long long)*slot);
@@ -261,6 +267,46 @@ totals_final_kernel(const double* partial, int nparts, double* totals)
     s = wave_sum(s);
     if (threadIdx.x == 0) totals[q] = s;
 }
+
+// MHD constrained transport helpers
+long mhd_workspace_bytes(int N, int precision)
+{
+    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
+    const long rs = precision == M3P_FP64 ? 8 : 4;
+    return (8 * N3 + 3 * 7 * nf) * rs + 256;
+}
+
+template<typename R>
+int mhd_advance(MhdArgs a, char* work, hipStream_t s)
+{
+    const int N = a.N;
+    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
+    R* P = reinterpret_cast<R*>(work);
+    R* F1 = P + 8 * N3;
+    R* F2 = F1 + 5 * nf;
+    R* F3 = F2 + 5 * nf;
+    R* E1 = F3 + 5 * nf;
+    R* E2 = E1 + 2 * nf;
+    R* E3 = E2 + 2 * nf;
+    char* tail = reinterpret_cast<char*>(E3 + 2 * nf);
+    tail += (256 - (reinterpret_cast<uintptr_t>(tail) & 255)) & 255;
+    a.vmax = reinterpret_cast<unsigned long long*>(tail);
+    if (!a.dt) a.dt = reinterpret_cast<double*>(tail + 8);
+    const int gc = (int)((N3 + 255) / 256), gf = (int)((nf + 255) / 256);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, a.vmax, 1, 0ull);
+    hipLaunchKernelGGL(mhd_prim_kernel<R>, dim3(gc), dim3(256), 0, s, a, P);
+    hipLaunchKernelGGL(mhd_dt_kernel, dim3(1), dim3(1), 0, s, a);
+    hipLaunchKernelGGL((mhd_face_kernel<R, 1>), dim3(gf), dim3(256), 0, s, a, (const R*)P, F1, E1);
+    hipLaunchKernelGGL((mhd_face_kernel<R, 2>), dim3(gf), dim3(256), 0, s, a, (const R*)P, F2, E2);
+    hipLaunchKernelGGL((mhd_face_kernel<R, 3>), dim3(gf), dim3(256), 0, s, a, (const R*)P, F3, E3);
+    hipLaunchKernelGGL(mhd_cell_update_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)F1, (const R*)F2, (const R*)F3);
+    FaceE<R> fe{ E1, E2, E3, N, nf };
+    hipLaunchKernelGGL((mhd_face_update_kernel<R, 1>), dim3(gf), dim3(256), 0, s, a, fe);
+    hipLaunchKernelGGL((mhd_face_update_kernel<R, 2>), dim3(gf), dim3(256), 0, s, a, fe);
+    hipLaunchKernelGGL((mhd_face_update_kernel<R, 3>), dim3(gf), dim3(256), 0, s, a, fe);
+    return launch_status("m3p_mhd_advance_ct");
+}
+
 
 } // namespace
 
@@ -484,6 +530,52 @@ int m3p_weighted_sum(const double* s, const double* t, double* out, long n, doub
     if (!s || !t || !out || n <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad weighted_sum arguments");
     hipLaunchKernelGGL(weighted_sum_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, s, t, out, n, b);
     return launch_status("weighted_sum_kernel");
+}
+
+long m3p_mhd_workspace_bytes(int N, int precision)
+{
+    if (N <= 0) return -1;
+    return mhd_workspace_bytes(N, precision);
+}
+
+int m3p_mhd_advance_ct(const double* uc, const double* bf1, const double* bf2, const double* bf3,
+                       const double* u0, const double* b01, const double* b02, const double* b03, double rk_b,
+                       int N, double dl, double cfl_number, double plm_theta, double gamma_law_index,
+                       double* uo, double* bo1, double* bo2, double* bo3, double* dt,
+                       void* work, long work_bytes, int precision, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!uc || !bf1 || !bf2 || !bf3 || !uo || !bo1 || !bo2 || !bo3 || !work || N < 2)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_advance_ct: bad arguments");
+    if (u0 && (!b01 || !b02 || !b03)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_advance_ct: RK base needs all four arrays");
+    if (uo == uc || bo1 == bf1 || bo2 == bf2 || bo3 == bf3)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_advance_ct: outputs must not alias inputs");
+    if (work_bytes < mhd_workspace_bytes(N, precision))
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_advance_ct: workspace too small (see m3p_mhd_workspace_bytes)");
+    MhdArgs a{};
+    a.uc = uc; a.b1 = bf1; a.b2 = bf2; a.b3 = bf3;
+    a.u0 = u0; a.b01 = b01; a.b02 = b02; a.b03 = b03; a.rk_b = rk_b;
+    a.uo = uo; a.bo1 = bo1; a.bo2 = bo2; a.bo3 = bo3;
+    a.N = N; a.dl = dl; a.cfl = cfl_number; a.theta = plm_theta; a.gamma = gamma_law_index;
+    a.dt = dt; a.err = ds->err;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == M3P_FP64) return mhd_advance<double>(a, (char*)work, s);
+    return mhd_advance<float>(a, (char*)work, s);
+}
+
+int m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, int N, double* out, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!bf1 || !bf2 || !bf3 || !out || N < 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_max_div_b: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const long N3 = (long)N * N * N;
+    unsigned long long* slot = ds->red + 3000;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, slot, 1, 0ull);
+    hipLaunchKernelGGL(mhd_divb_kernel, dim3((N3 + 255) / 256), dim3(256), 0, s, bf1, bf2, bf3, N, slot);
+    hipLaunchKernelGGL(copy_u64_as_double_kernel, dim3(1), dim3(1), 0, s, slot, out);
+    return launch_status("m3p_mhd_max_div_b");
 }
 
 int m3p_set_stage2d_tile(int kind, int precision, int tj, int h)

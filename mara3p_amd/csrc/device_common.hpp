@@ -40,6 +40,23 @@ __device__ __forceinline__ R plm(R yl, R y0, R yr, R theta)
     }
 }
 
+// Half the PLM slope, 0.5 * plm(...), used for the face states p -/+ 0.5 g.
+// FP64: the reference form times 0.5 (exact).  FP32: the same value from
+// three-operand min/max: a common sign gives sign * min|.|, else 0.
+template<typename R>
+__device__ __forceinline__ R plm_half(R yl, R y0, R yr, R theta)
+{
+    if constexpr (is_fp64<R>::value) {
+        return R(0.5) * plm(yl, y0, yr, theta);
+    } else {
+        R ht = R(0.5) * theta;
+        R a = (y0 - yl) * ht;
+        R b = (yr - yl) * R(0.25);
+        R c = (yr - y0) * ht;
+        return fmaxf(R(0), fminf(fminf(a, b), c)) + fminf(R(0), fmaxf(fmaxf(a, b), c));
+    }
+}
+
 // Division / square root: IEEE for FP64 (parity), fast for FP32 (mixed).
 template<typename R> __device__ __forceinline__ R fdiv(R a, R b)
 {

@@ -1,0 +1,418 @@
+// Ideal MHD with constrained transport on a periodic N^3 cube:
+// mara::advance_mhd_ct (src/scheme_mhd_v1.cpp:265-299) as four kernels.
+//
+//   1. mhd_prim_kernel     cell primitives from U and the face-averaged B
+//                          (primitive_array :214-224, face_to_cell
+//                          src/mesh_cartesian_3d.hpp:235-245) + CFL max speed
+//   2. mhd_face_kernel<A>  PLM face states along axis A, HLLE Godunov flux and
+//                          face electric field (godunov_fluxes_plm :152-179,
+//                          mhd::riemann_hlle src/physics_mhd.hpp:358-390)
+//   3. mhd_cell_update     U - div(F) dt / dl (divergence_difference
+//                          src/mesh_cartesian_3d.hpp:290-301)
+//   4. mhd_face_update<A>  B_f - curl(E) dt / dl with edge EMFs averaged from
+//                          the faces (edge_emf_from_face :76-98,
+//                          solenoidal_difference src/mesh_cartesian_3d.hpp:265-276)
+//
+// Layouts (row-major, last axis fastest, as the reference arrays):
+//   U [5][N][N][N];  B1 [N+1][N][N], B2 [N][N+1][N], B3 [N][N][N+1]
+//   workspace: P [8][N^3], face fluxes F_A [5][nf], face fields E_A [2][nf]
+//   (nf = (N+1) N^2, the transverse E components of axis-A faces).
+#pragma once
+#include "device_common.hpp"
+
+namespace m3p {
+
+template<typename R>
+struct Mhd {
+    __device__ static __forceinline__ R dot3(const R* a, const R* b)
+    {
+        if constexpr (is_fp64<R>::value) return ((R(0) + a[0] * b[0]) + a[1] * b[1]) + a[2] * b[2];
+        else return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0]));
+    }
+
+    // conserved_density :307-319
+    __device__ static __forceinline__ void cons(const R* p, R gamma, R* u)
+    {
+        const R* v = &p[1];
+        const R* b = &p[5];
+        R d = p[0];
+        u[0] = d;
+        u[1] = d * v[0];
+        u[2] = d * v[1];
+        u[3] = d * v[2];
+        if constexpr (is_fp64<R>::value) u[4] = R(0.5) * d * dot3(v, v) + p[4] / (gamma - R(1)) + R(0.5) * dot3(b, b);
+        else u[4] = fmaf(R(0.5) * d, dot3(v, v), fmaf(R(0.5), dot3(b, b), p[4] * frcp(gamma - R(1))));
+    }
+
+    // flux :324-347 along unit axis AXIS (1, 2, 3)
+    template<int AXIS>
+    __device__ static __forceinline__ void flux(const R* p, const R* u, R* f)
+    {
+        const R* v = &p[1];
+        const R* b = &p[5];
+        constexpr int a = AXIS - 1;
+        if constexpr (is_fp64<R>::value) {
+            R bn = R(0) + b[a];
+            R vn = R(0) + v[a];
+            R pt = p[4] + R(0.5) * dot3(b, b);
+            R bv = dot3(b, v);
+            R n[3] = { R(AXIS == 1), R(AXIS == 2), R(AXIS == 3) };
+            f[0] = vn * u[0] + R(0) + R(0);
+            f[1] = vn * u[1] + pt * n[0] + -bn * b[0];
+            f[2] = vn * u[2] + pt * n[1] + -bn * b[1];
+            f[3] = vn * u[3] + pt * n[2] + -bn * b[2];
+            f[4] = vn * u[4] + pt * vn + -bn * bv;
+        } else {
+            R bn = b[a], vn = v[a];
+            R pt = fmaf(R(0.5), dot3(b, b), p[4]);
+            f[0] = vn * u[0];
+            f[1] = fmaf(vn, u[1], -bn * b[0]);
+            f[2] = fmaf(vn, u[2], -bn * b[1]);
+            f[3] = fmaf(vn, u[3], -bn * b[2]);
+            f[1 + a] += pt;
+            f[4] = fmaf(vn, u[4] + pt, -bn * dot3(b, v));
+        }
+    }
+
+    // induction :349-352 = n x E with E = -(v x B) (electric_field_vector :154-157)
+    template<int AXIS>
+    __device__ static __forceinline__ void induction(const R* p, R* out)
+    {
+        const R* v = &p[1];
+        const R* b = &p[5];
+        R E[3] = { -(v[1] * b[2] - v[2] * b[1]), -(v[2] * b[0] - v[0] * b[2]), -(v[0] * b[1] - v[1] * b[0]) };
+        R n[3] = { R(AXIS == 1), R(AXIS == 2), R(AXIS == 3) };
+        out[0] = n[1] * E[2] - n[2] * E[1];
+        out[1] = n[2] * E[0] - n[0] * E[2];
+        out[2] = n[0] * E[1] - n[1] * E[0];
+    }
+
+    // outer_wavespeeds :216-246
+    template<int AXIS>
+    __device__ static __forceinline__ void wavespeeds(const R* p, R gamma, R& wm, R& wp)
+    {
+        constexpr int a = AXIS - 1;
+        const R* b = &p[5];
+        R d = p[0];
+        if constexpr (is_fp64<R>::value) {
+            R vn = R(0) + p[1 + a];
+            R cs2 = gamma * p[4] / d;
+            R ca2 = dot3(b, b) / d;
+            R cf2 = cs2 + ca2;
+            R bn = R(0) + b[a];
+            R can2 = pow(bn, R(2)) / d;
+            R cg2 = sqrt(cf2 * cf2 - R(4) * cs2 * can2);
+            R cf = sqrt(R(0.5) * (cf2 + cg2));
+            wm = vn - cf;
+            wp = vn + cf;
+        } else {
+            R vn = p[1 + a];
+            R id = frcp(d);
+            R cs2 = gamma * p[4] * id;
+            R ca2 = dot3(b, b) * id;
+            R cf2 = cs2 + ca2;
+            R can2 = b[a] * b[a] * id;
+            R cg2 = fsqrt(fmaxf(fmaf(cf2, cf2, -R(4) * cs2 * can2), R(0)));
+            R cf = fsqrt(R(0.5) * (cf2 + cg2));
+            wm = vn - cf;
+            wp = vn + cf;
+        }
+    }
+
+    // riemann_hlle :358-390 -> Godunov flux f[5], face electric field e[3]
+    template<int AXIS>
+    __device__ static __forceinline__ void riemann(R* pl, R* pr, R bl, R gamma, R* f, R* e)
+    {
+        constexpr int a = AXIS - 1;
+        // with_longitudinal_field :165-176: (b - (b.n) n) + bl n
+        if constexpr (is_fp64<R>::value) {
+            R bnl = R(0) + pl[5 + a], bnr = R(0) + pr[5 + a];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R nk = R(k == a);
+                pl[5 + k] = (pl[5 + k] - bnl * nk) + bl * nk;
+                pr[5 + k] = (pr[5 + k] - bnr * nk) + bl * nk;
+            }
+        } else {
+            pl[5 + a] = bl;
+            pr[5 + a] = bl;
+        }
+        R ul[5], ur[5], fl[5], fr[5], il[3], ir[3], alm, alp, arm, arp;
+        cons(pl, gamma, ul);
+        cons(pr, gamma, ur);
+        flux<AXIS>(pl, ul, fl);
+        flux<AXIS>(pr, ur, fr);
+        induction<AXIS>(pl, il);
+        induction<AXIS>(pr, ir);
+        wavespeeds<AXIS>(pl, gamma, alm, alp);
+        wavespeeds<AXIS>(pr, gamma, arm, arp);
+        R ap = smax(R(0), smax(alp, arp));
+        R am = smin(R(0), smin(alm, arm));
+        R hi[3];
+        if constexpr (is_fp64<R>::value) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) / (ap - am);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) hi[k] = (ap * il[k] - am * ir[k] - (pl[5 + k] - pr[5 + k]) * ap * am) / (ap - am);
+        } else {
+            R inv = frcp(ap - am), apam = ap * am;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) f[k] = fmaf(ap, fl[k], fmaf(-am, fr[k], -(ul[k] - ur[k]) * apam)) * inv;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) hi[k] = fmaf(ap, il[k], fmaf(-am, ir[k], -(pl[5 + k] - pr[5 + k]) * apam)) * inv;
+        }
+        // -n x hll_induction
+        R n[3] = { R(AXIS == 1), R(AXIS == 2), R(AXIS == 3) };
+        e[0] = -(n[1] * hi[2] - n[2] * hi[1]);
+        e[1] = -(n[2] * hi[0] - n[0] * hi[2]);
+        e[2] = -(n[0] * hi[1] - n[1] * hi[0]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// recover_primitive :287-305 (FP64), fastest_speed :248-258 (FP64)
+__device__ __forceinline__ int mhd_recover(const double* u, const double* b, double gamma, double* p)
+{
+    double d = u[0];
+    double S[3] = { u[1], u[2], u[3] };
+    double ke = 0.5 * Mhd<double>::dot3(S, S) / d;
+    double me = 0.5 * Mhd<double>::dot3(b, b);
+    double pg = (u[4] - ke - me) * (gamma - 1);
+    p[0] = d; p[1] = S[0] / d; p[2] = S[1] / d; p[3] = S[2] / d; p[4] = pg;
+    p[5] = b[0]; p[6] = b[1]; p[7] = b[2];
+    if (d < 0.0) return 1;
+    if (pg < 0.0) return 2;
+    return 0;
+}
+
+__device__ __forceinline__ double mhd_fastest_speed(const double* p, double gamma)
+{
+    double m1, q1, m2, q2, m3, q3;
+    Mhd<double>::wavespeeds<1>(p, gamma, m1, q1);
+    Mhd<double>::wavespeeds<2>(p, gamma, m2, q2);
+    Mhd<double>::wavespeeds<3>(p, gamma, m3, q3);
+    return smax(smax(smax(fabs(m1), fabs(q1)), smax(fabs(m2), fabs(q2))), smax(fabs(m3), fabs(q3)));
+}
+
+struct MhdArgs {
+    const double* uc; const double* b1; const double* b2; const double* b3;
+    const double* u0; const double* b01; const double* b02; const double* b03;   // RK base or null
+    double* uo; double* bo1; double* bo2; double* bo3;
+    double rk_b;
+    int N;
+    double dl, cfl, theta, gamma;
+    double* dt;                      // device: CFL time step of this call (written by mhd_dt_kernel)
+    unsigned long long* vmax;        // device: max fastest speed (as ordered bits)
+    int* err;
+};
+
+__device__ __forceinline__ int wrapi(int i, int N) { return i < 0 ? i + N : (i >= N ? i - N : i); }
+
+template<typename R>
+__global__ void __launch_bounds__(256)
+mhd_prim_kernel(MhdArgs a, R* P)
+{
+    __shared__ double part[4];
+    const int N = a.N;
+    const long N3 = (long)N * N * N;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (c < N3) {
+        const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);
+        const long NN = (long)N * N;
+        double b[3] = { 0.5 * (a.b1[(long)i * NN + (long)j * N + k] + a.b1[(long)(i + 1) * NN + (long)j * N + k]),
+                        0.5 * (a.b2[((long)i * (N + 1) + j) * N + k] + a.b2[((long)i * (N + 1) + j + 1) * N + k]),
+                        0.5 * (a.b3[((long)i * N + j) * (N + 1) + k] + a.b3[((long)i * N + j) * (N + 1) + k + 1]) };
+        double u[5], p[8];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) u[q] = a.uc[q * N3 + c];
+        int st = mhd_recover(u, b, a.gamma, p);
+        if (st) report_error(a.err, st == 1 ? 1 : 2, 0, (int)c);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) P[q * N3 + c] = R(p[q]);
+        s = mhd_fastest_speed(p, a.gamma);
+    }
+    s = wave_max(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = part[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, part[w]);
+        atomicMax(a.vmax, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+__global__ void mhd_dt_kernel(MhdArgs a)
+{
+    double vmax = __longlong_as_double((long long)*a.vmax);
+    *a.dt = a.cfl * a.dl / vmax;
+}
+
+// Faces of axis A: index (i, j, k) with N+1 entries along A.  Cells along the
+// axis are (m-2, m-1, m, m+1) for face m, wrapped periodically.
+template<typename R, int A>
+__global__ void __launch_bounds__(256)
+mhd_face_kernel(MhdArgs a, const R* P, R* F, R* E)
+{
+    const int N = a.N;
+    const long N3 = (long)N * N * N;
+    const long nf = (long)(N + 1) * N * N;
+    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    int i, j, k;
+    if (A == 1)      { i = (int)(f / ((long)N * N)); j = (int)(f / N % N); k = (int)(f % N); }
+    else if (A == 2) { i = (int)(f / ((long)(N + 1) * N)); j = (int)(f / N % (N + 1)); k = (int)(f % N); }
+    else             { i = (int)(f / ((long)N * (N + 1))); j = (int)(f / (N + 1) % N); k = (int)(f % (N + 1)); }
+    long cell[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int o = s - 2;
+        int ci = i, cj = j, ck = k;
+        if (A == 1) ci = wrapi(i + o, N);
+        if (A == 2) cj = wrapi(j + o, N);
+        if (A == 3) ck = wrapi(k + o, N);
+        cell[s] = ((long)ci * N + cj) * N + ck;
+    }
+    const R theta = R(a.theta);
+    R pl[8], pr[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        R c0 = P[q * N3 + cell[0]], c1 = P[q * N3 + cell[1]], c2 = P[q * N3 + cell[2]], c3 = P[q * N3 + cell[3]];
+        pl[q] = c1 + R(0.5) * plm(c0, c1, c2, theta);
+        pr[q] = c2 - R(0.5) * plm(c1, c2, c3, theta);
+    }
+    const double* bf = A == 1 ? a.b1 : (A == 2 ? a.b2 : a.b3);
+    R fl[5], e[3];
+    Mhd<R>::template riemann<A>(pl, pr, R(bf[f]), R(a.gamma), fl, e);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) F[q * nf + f] = fl[q];
+    // keep the two transverse components: A=1 -> (Ey, Ez), A=2 -> (Ex, Ez), A=3 -> (Ex, Ey)
+    E[f] = e[A == 1 ? 1 : 0];
+    E[nf + f] = e[A == 3 ? 1 : 2];
+}
+
+// Face-centred transverse electric field of axis-A faces, by component.
+template<typename R>
+struct FaceE {
+    const R* E1; const R* E2; const R* E3;   // each [2][nf]
+    int N;
+    long nf;
+    __device__ __forceinline__ double x_face(int c, int i, int j, int k) const   // c in {1:y, 2:z}
+    {
+        return double(E1[(c - 1) * nf + ((long)i * N + wrapi(j, N)) * N + wrapi(k, N)]);
+    }
+    __device__ __forceinline__ double y_face(int c, int i, int j, int k) const   // c in {0:x, 2:z}
+    {
+        return double(E2[(c == 0 ? 0 : 1) * nf + ((long)wrapi(i, N) * (N + 1) + j) * N + wrapi(k, N)]);
+    }
+    __device__ __forceinline__ double z_face(int c, int i, int j, int k) const   // c in {0:x, 1:y}
+    {
+        return double(E3[c * nf + ((long)wrapi(i, N) * N + wrapi(j, N)) * (N + 1) + k]);
+    }
+    // edge_emf_from_face :76-98
+    // x-edge at (x cell i, y face j, z face k)
+    __device__ __forceinline__ double ee1(int i, int j, int k) const
+    {
+        double a = 0.5 * (y_face(0, i, j, k - 1) + y_face(0, i, j, k));
+        double b = 0.5 * (z_face(0, i, j - 1, k) + z_face(0, i, j, k));
+        return 0.5 * (a + b);
+    }
+    // y-edge at (x face i, y cell j, z face k)
+    __device__ __forceinline__ double ee2(int i, int j, int k) const
+    {
+        double a = 0.5 * (z_face(1, i - 1, j, k) + z_face(1, i, j, k));
+        double b = 0.5 * (x_face(1, i, j, k - 1) + x_face(1, i, j, k));
+        return 0.5 * (a + b);
+    }
+    // z-edge at (x face i, y face j, z cell k)
+    __device__ __forceinline__ double ee3(int i, int j, int k) const
+    {
+        double a = 0.5 * (x_face(2, i, j - 1, k) + x_face(2, i, j, k));
+        double b = 0.5 * (y_face(2, i - 1, j, k) + y_face(2, i, j, k));
+        return 0.5 * (a + b);
+    }
+};
+
+template<typename R>
+__global__ void __launch_bounds__(256)
+mhd_cell_update_kernel(MhdArgs a, const R* F1, const R* F2, const R* F3)
+{
+    const int N = a.N;
+    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N3) return;
+    const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);
+    const long x0 = ((long)i * N + j) * N + k, x1 = x0 + (long)N * N;
+    const long y0 = ((long)i * (N + 1) + j) * N + k, y1 = y0 + N;
+    const long z0 = ((long)i * N + j) * (N + 1) + k, z1 = z0 + 1;
+    const double dt = *a.dt;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        double d1 = double(F1[q * nf + x1]) - double(F1[q * nf + x0]);
+        double d2 = double(F2[q * nf + y1]) - double(F2[q * nf + y0]);
+        double d3 = double(F3[q * nf + z1]) - double(F3[q * nf + z0]);
+        double r = a.uc[q * N3 + c] - (d1 + d2 + d3) * dt / a.dl;
+        if (a.u0) r = a.u0[q * N3 + c] * a.rk_b + r * (1.0 - a.rk_b);
+        a.uo[q * N3 + c] = r;
+    }
+}
+
+template<typename R, int A>
+__global__ void __launch_bounds__(256)
+mhd_face_update_kernel(MhdArgs a, FaceE<R> fe)
+{
+    const int N = a.N;
+    const long nf = (long)(N + 1) * N * N;
+    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const double dt = *a.dt;
+    double cf, b, b0 = 0.0;
+    if (A == 1) {
+        const int i = (int)(f / ((long)N * N)), j = (int)(f / N % N), k = (int)(f % N);
+        cf = (fe.ee3(i, j + 1, k) - fe.ee3(i, j, k)) - (fe.ee2(i, j, k + 1) - fe.ee2(i, j, k));
+        b = a.b1[f];
+        if (a.u0) b0 = a.b01[f];
+    } else if (A == 2) {
+        const int i = (int)(f / ((long)(N + 1) * N)), j = (int)(f / N % (N + 1)), k = (int)(f % N);
+        cf = (fe.ee1(i, j, k + 1) - fe.ee1(i, j, k)) - (fe.ee3(i + 1, j, k) - fe.ee3(i, j, k));
+        b = a.b2[f];
+        if (a.u0) b0 = a.b02[f];
+    } else {
+        const int i = (int)(f / ((long)N * (N + 1))), j = (int)(f / (N + 1) % N), k = (int)(f % (N + 1));
+        cf = (fe.ee2(i + 1, j, k) - fe.ee2(i, j, k)) - (fe.ee1(i, j + 1, k) - fe.ee1(i, j, k));
+        b = a.b3[f];
+        if (a.u0) b0 = a.b03[f];
+    }
+    double r = b - cf * dt / a.dl;
+    if (a.u0) r = b0 * a.rk_b + r * (1.0 - a.rk_b);
+    double* out = A == 1 ? a.bo1 : (A == 2 ? a.bo2 : a.bo3);
+    out[f] = r;
+}
+
+// max over cells of |div B| = |d1(B1) + d2(B2) + d3(B3)| (divergence_difference)
+__global__ void __launch_bounds__(256)
+mhd_divb_kernel(const double* b1, const double* b2, const double* b3, int N, unsigned long long* slot)
+{
+    __shared__ double part[4];
+    const long N3 = (long)N * N * N;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (c < N3) {
+        const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);
+        const long NN = (long)N * N;
+        double d1 = b1[(long)(i + 1) * NN + (long)j * N + k] - b1[(long)i * NN + (long)j * N + k];
+        double d2 = b2[((long)i * (N + 1) + j + 1) * N + k] - b2[((long)i * (N + 1) + j) * N + k];
+        double d3 = b3[((long)i * N + j) * (N + 1) + k + 1] - b3[((long)i * N + j) * (N + 1) + k];
+        v = fabs(d1 + d2 + d3);
+    }
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = part[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, part[w]);
+        atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+} // namespace m3p

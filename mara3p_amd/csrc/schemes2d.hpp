@@ -23,6 +23,7 @@ struct TileCtx {
 template<typename R>
 struct EulerScheme {
     static constexpr int NC = 5;
+    static constexpr int kWavesPerEU = is_fp64<R>::value ? 2 : 4;   // occupancy floor (VGPR budget)
     using Real = R;
     using Params = EulerParams;
     template<int H, int TJ> struct TileData {};
@@ -79,6 +80,7 @@ struct MinidiskParams {
 template<typename R>
 struct MinidiskScheme {
     static constexpr int NC = 3;
+    static constexpr int kWavesPerEU = is_fp64<R>::value ? 3 : 5;
     using Real = R;
     using Params = MinidiskParams;
     // Vertex coordinates of the tile's rows/columns in FP64 (reference formula),
@@ -247,6 +249,7 @@ struct IsoTableParams {
 template<typename R>
 struct IsoTableScheme {
     static constexpr int NC = 3;
+    static constexpr int kWavesPerEU = 2;
     using Real = R;
     using Params = IsoTableParams;
     template<int H, int TJ> struct TileData {};

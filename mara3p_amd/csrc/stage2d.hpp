@@ -133,7 +133,7 @@ struct StageArgs {
 // its load to its update (no second read of the state); otherwise re-load it
 // at the update (fewer registers).
 template<class S, class Ld, int H, int TJ, bool KEEPU>
-__global__ void __launch_bounds__(TJ)
+__global__ void __launch_bounds__(TJ) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU, 8)))
 stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
 {
     constexpr int NC = S::NC;
@@ -159,11 +159,11 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
 
     __shared__ typename S::template TileData<H, TJ> td;
     __shared__ R hraw[7][NC][H];     // pre-pass primitives at 7 edge columns
-    __shared__ R hG[NC][H];          // y slope at column c0-1
+    __shared__ R hG[NC][H];          // y half slope at column c0-1
     __shared__ R hF[NC][H];          // y flux at face c0+TJ
-    __shared__ R rowP[NC][TJ + 2];   // [0] = c0-1, [t+1] = c0+t, [TJ+1] = c0+TJ
-    __shared__ R rowG[NC][TJ + 1];   // [0] = c0-1, [t+1] = c0+t
-    __shared__ R rowF[NC][TJ + 1];   // [t] = face c0+t, [TJ] = face c0+TJ
+    __shared__ R rowP[NC][TJ + 2];   // [t+1] = column c0+t (edges come from hraw)
+    __shared__ R rowG[NC][TJ + 1];   // [t+1] = half slope at column c0+t (left edge from hG)
+    __shared__ R rowF[NC][TJ + 1];   // [t] = y flux at face c0+t (right edge from hF)
 
     S::setup(td, prm, tc);
     bool ok = true;
@@ -183,11 +183,11 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
         R pl[NC], pr[NC], f[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-            hG[m][r] = plm(hraw[0][m][r], hraw[1][m][r], hraw[2][m][r], theta);
-            R g1 = plm(hraw[3][m][r], hraw[4][m][r], hraw[5][m][r], theta);
-            R g2 = plm(hraw[4][m][r], hraw[5][m][r], hraw[6][m][r], theta);
-            pl[m] = hraw[4][m][r] + R(0.5) * g1;
-            pr[m] = hraw[5][m][r] - R(0.5) * g2;
+            hG[m][r] = plm_half(hraw[0][m][r], hraw[1][m][r], hraw[2][m][r], theta);
+            R g1 = plm_half(hraw[3][m][r], hraw[4][m][r], hraw[5][m][r], theta);
+            R g2 = plm_half(hraw[4][m][r], hraw[5][m][r], hraw[6][m][r], theta);
+            pl[m] = hraw[4][m][r] + g1;
+            pr[m] = hraw[5][m][r] - g2;
         }
         S::template face<2>(pl, pr, prm, td, tc, tc.k0 + r, tc.c0 + TJ, f);
 #pragma unroll
@@ -195,7 +195,8 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
     }
 
     // ---- x warm-up: rows k0-2 .. k0+1, flux at face k0 ------------------
-    R Pa[NC], Pb[NC], ga[NC], Fxp[NC];
+    R Pa[NC], Pb[NC], ga[NC];   // ga: half slope at the current row
+    double Fxp[NC];             // x flux at the current row's lower face
     double Ua[NC], Ub[NC];
     {
         R Pm2[NC], Pm1[NC], gm[NC], pl[NC], pr[NC];
@@ -210,16 +211,21 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
         }
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-            gm[m] = plm(Pm2[m], Pm1[m], Pa[m], theta);
-            ga[m] = plm(Pm1[m], Pa[m], Pb[m], theta);
-            pl[m] = Pm1[m] + R(0.5) * gm[m];
-            pr[m] = Pa[m] - R(0.5) * ga[m];
+            gm[m] = plm_half(Pm2[m], Pm1[m], Pa[m], theta);
+            ga[m] = plm_half(Pm1[m], Pa[m], Pb[m], theta);
+            pl[m] = Pm1[m] + gm[m];
+            pr[m] = Pa[m] - ga[m];
         }
-        S::template face<1>(pl, pr, prm, td, tc, tc.k0, c, Fxp);
+        R f[NC];
+        S::template face<1>(pl, pr, prm, td, tc, tc.k0, c, f);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) Fxp[m] = double(f[m]);
     }
     __syncthreads();
 
     // ---- march ------------------------------------------------------------
+    // Edge lanes read their out-of-tile neighbours from the pre-pass arrays
+    // through a selected address (no divergent branch).
     for (int kk = 0; kk < H; ++kk) {
         const int k = tc.k0 + kk;
         R Pc[NC], gb[NC], pl[NC], pr[NC], Fxn[NC], gy[NC], Fy[NC];
@@ -228,44 +234,42 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
         else                 ok &= ld.prim_row(b, k + 2, c, prm, Pc);
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-            gb[m] = plm(Pa[m], Pb[m], Pc[m], theta);
-            pl[m] = Pa[m] + R(0.5) * ga[m];
-            pr[m] = Pb[m] - R(0.5) * gb[m];
+            gb[m] = plm_half(Pa[m], Pb[m], Pc[m], theta);
+            pl[m] = Pa[m] + ga[m];
+            pr[m] = Pb[m] - gb[m];
         }
         S::template face<1>(pl, pr, prm, td, tc, k + 1, c, Fxn);
 
 #pragma unroll
         for (int m = 0; m < NC; ++m) rowP[m][t + 1] = Pa[m];
-        if (t == 0) {
-#pragma unroll
-            for (int m = 0; m < NC; ++m) { rowP[m][0] = hraw[1][m][kk]; rowG[m][0] = hG[m][kk]; }
-        }
-        if (t == TJ - 1) {
-#pragma unroll
-            for (int m = 0; m < NC; ++m) { rowP[m][TJ + 1] = hraw[5][m][kk]; rowF[m][TJ] = hF[m][kk]; }
-        }
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-            gy[m] = plm(rowP[m][t], Pa[m], rowP[m][t + 2], theta);
+            const R* left = t == 0 ? &hraw[1][m][kk] : &rowP[m][t];
+            const R* right = t == TJ - 1 ? &hraw[5][m][kk] : &rowP[m][t + 2];
+            gy[m] = plm_half(*left, Pa[m], *right, theta);
             rowG[m][t + 1] = gy[m];
         }
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-            pl[m] = rowP[m][t] + R(0.5) * rowG[m][t];
-            pr[m] = Pa[m] - R(0.5) * gy[m];
+            const R* lp = t == 0 ? &hraw[1][m][kk] : &rowP[m][t];
+            const R* lg = t == 0 ? &hG[m][kk] : &rowG[m][t];
+            pl[m] = *lp + *lg;
+            pr[m] = Pa[m] - gy[m];
         }
         S::template face<2>(pl, pr, prm, td, tc, k, c, Fy);
 #pragma unroll
         for (int m = 0; m < NC; ++m) rowF[m][t] = Fy[m];
         __syncthreads();
 
-        double dfx[NC], dfy[NC], uu[NC], res[NC];
+        double dfx[NC], dfy[NC], uu[NC], res[NC], fxn[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-            dfx[m] = double(Fxn[m]) - double(Fxp[m]);
-            dfy[m] = double(rowF[m][t + 1]) - double(Fy[m]);
+            const R* rf = t == TJ - 1 ? &hF[m][kk] : &rowF[m][t + 1];
+            fxn[m] = double(Fxn[m]);
+            dfx[m] = fxn[m] - Fxp[m];
+            dfy[m] = double(*rf) - double(Fy[m]);
         }
         if constexpr (KEEPU) {
 #pragma unroll
@@ -284,7 +288,7 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
             for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = res[m];
         }
 #pragma unroll
-        for (int m = 0; m < NC; ++m) { Pa[m] = Pb[m]; Pb[m] = Pc[m]; ga[m] = gb[m]; Fxp[m] = Fxn[m]; }
+        for (int m = 0; m < NC; ++m) { Pa[m] = Pb[m]; Pb[m] = Pc[m]; ga[m] = gb[m]; Fxp[m] = fxn[m]; }
         if constexpr (KEEPU) {
 #pragma unroll
             for (int m = 0; m < NC; ++m) { Ua[m] = Ub[m]; Ub[m] = Uc[m]; }

@@ -144,3 +144,42 @@ def sod(nblk=16, block_size=64, gamma=1.4):
     z = np.zeros(N)
     u = euler_conserved(rho, z, z, z, p, gamma)        # [N, 5]
     return np.ascontiguousarray(u.reshape(nblk, n, 5).transpose(0, 2, 1))
+
+
+def orszag_tang_3d(N, seed=1, perturbation=1e-3, gamma=5.0 / 3.0):
+    """3D Orszag-Tang-style state on the periodic unit cube with N^3 cells:
+    rho = 25/(36 pi), p = 5/(12 pi), v = (-sin 2 pi y, sin 2 pi x, vz),
+    B = (-sin 2 pi y, sin 4 pi x, 0) / sqrt(4 pi), vz = perturbation * U(-1, 1)
+    seeded per cell.  Face fields come from the vector potential
+    A_z = (cos(2 pi y) / (2 pi) + cos(4 pi x) / (4 pi)) / sqrt(4 pi) line-integrated
+    on the z edges and differenced (as construct_conserved,
+    src/scheme_mhd_v1.cpp:305-342), so div B = 0 to round-off.
+    Returns (uc [5][N][N][N], bf1 [N+1][N][N], bf2 [N][N+1][N], bf3 [N][N][N+1])."""
+    dl = 1.0 / N
+    xv = np.arange(N + 1) * dl
+    xc = (np.arange(N) + 0.5) * dl
+    s4 = math.sqrt(4.0 * math.pi)
+
+    def Az(x, y):
+        return (np.cos(2 * math.pi * y) / (2 * math.pi) + np.cos(4 * math.pi * x) / (4 * math.pi)) / s4
+    # x faces at (xv_i, y cell j): B1 = (Az(x_i, y_{j+1}) - Az(x_i, y_j)) / dl
+    a = Az(xv[:, None], xv[None, :])                       # [N+1][N+1] at (x vertex, y vertex)
+    b1 = (a[:, 1:] - a[:, :-1]) / dl                        # [N+1][N]
+    b2 = -(a[1:, :] - a[:-1, :]) / dl                       # [N][N+1]
+    bf1 = np.ascontiguousarray(np.broadcast_to(b1[:, :, None], (N + 1, N, N)))
+    bf2 = np.ascontiguousarray(np.broadcast_to(b2[:, :, None], (N, N + 1, N)))
+    bf3 = np.zeros((N, N, N + 1))
+    bx = 0.5 * (bf1[1:] + bf1[:-1])
+    by = 0.5 * (bf2[:, 1:] + bf2[:, :-1])
+    bz = 0.5 * (bf3[:, :, 1:] + bf3[:, :, :-1])
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    rho = np.full((N, N, N), 25.0 / (36.0 * math.pi))
+    p = np.full((N, N, N), 5.0 / (12.0 * math.pi))
+    vx = np.broadcast_to(-np.sin(2 * math.pi * Y)[:, :, None], (N, N, N))
+    vy = np.broadcast_to(np.sin(2 * math.pi * X)[:, :, None], (N, N, N))
+    idx = np.arange(N * N * N, dtype=np.uint64).reshape(N, N, N)
+    vz = perturbation * (2.0 * uniform01(seed, idx) - 1.0)
+    v2 = vx * vx + vy * vy + vz * vz
+    E = 0.5 * rho * v2 + p / (gamma - 1.0) + 0.5 * (bx * bx + by * by + bz * bz)
+    uc = np.stack([rho, rho * vx, rho * vy, rho * vz, E])
+    return np.ascontiguousarray(uc), bf1, bf2, bf3

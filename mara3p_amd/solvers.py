@@ -153,3 +153,56 @@ class Euler1DSolver(_StateSolver):
         self.stage(self.u, None, self.u1, dt)
         self.stage(self.u1, self.u, self.u2, dt)
         self.u, self.u2 = self.u2, self.u
+
+
+class MhdSolver:
+    """Ideal MHD with constrained transport on a periodic N^3 cube:
+    mara::advance_mhd_ct (src/scheme_mhd_v1.cpp:265-299) per RK stage, RK2
+    with the stage-wise CFL time step of the reference (examples/mhd3d.cpp:186-216)."""
+
+    def __init__(self, N, cfl=0.33, plm_theta=1.5, gamma=5.0 / 3.0, precision="fp64", device="cuda"):
+        self.N, self.cfl, self.theta, self.gamma = N, cfl, plm_theta, gamma
+        self.precision = PRECISIONS[precision]
+        self.dl = 1.0 / N
+        self.device = torch.device(device)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        shapes = [(5, N, N, N), (N + 1, N, N), (N, N + 1, N), (N, N, N + 1)]
+        self.state_ = [torch.zeros(s, **f64) for s in shapes]
+        self.stage_ = [torch.zeros(s, **f64) for s in shapes]
+        self.next_ = [torch.zeros(s, **f64) for s in shapes]
+        nbytes = lib().m3p_mhd_workspace_bytes(N, self.precision)
+        self.work = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        self.dts = torch.zeros(2, **f64)
+        self.divb = torch.zeros(1, **f64)
+        self.time = 0.0
+
+    def set_state(self, uc, bf1, bf2, bf3):
+        for t, a in zip(self.state_, (uc, bf1, bf2, bf3)):
+            t.copy_(torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64))
+
+    def state(self):
+        return [t.cpu().numpy() for t in self.state_]
+
+    def advance(self, src, dst, base=None, rk_b=0.5, dt_slot=0):
+        b = base if base is not None else [None] * 4
+        dt = self.dts[dt_slot:dt_slot + 1]
+        check(lib().m3p_mhd_advance_ct(_ptr(src[0]), _ptr(src[1]), _ptr(src[2]), _ptr(src[3]),
+                                       _ptr(b[0]), _ptr(b[1]), _ptr(b[2]), _ptr(b[3]), rk_b,
+                                       self.N, self.dl, self.cfl, self.theta, self.gamma,
+                                       _ptr(dst[0]), _ptr(dst[1]), _ptr(dst[2]), _ptr(dst[3]), _ptr(dt),
+                                       _ptr(self.work), self.work.numel(), self.precision, _stream()))
+
+    def step(self):
+        """One RK2 step (control::advance_runge_kutta order 2); returns nothing,
+        the two stage time steps stay in self.dts (device)."""
+        self.advance(self.state_, self.stage_, dt_slot=0)
+        self.advance(self.stage_, self.next_, base=self.state_, dt_slot=1)
+        self.state_, self.next_ = self.next_, self.state_
+
+    def max_div_b(self):
+        s = self.state_
+        check(lib().m3p_mhd_max_div_b(_ptr(s[1]), _ptr(s[2]), _ptr(s[3]), self.N, _ptr(self.divb), _stream()))
+        return float(self.divb.item())
+
+    def check_errors(self):
+        _lib.check_kernel_errors(torch.cuda.current_stream().cuda_stream)

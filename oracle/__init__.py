@@ -59,6 +59,13 @@ _SIG = {
     "orc_minidisk_mesh_rk2": (_I, [_P, ctypes.POINTER(OrcMinidisk), _D, _D, _I, _I]),
     "orc_minidisk_mesh_stage": (_I, [_P, _P, ctypes.POINTER(OrcMinidisk), _D, _D, _I]),
     "orc_euler_exact_riemann": (None, [_P, _P, _D, _D, _P]),
+    "orc_mhd_recover_primitive": (_I, [_P, _P, _D, _P]),
+    "orc_mhd_conserved_density": (None, [_P, _D, _P]),
+    "orc_mhd_flux": (None, [_P, _I, _D, _P]),
+    "orc_mhd_fastest_speed": (_D, [_P, _D]),
+    "orc_mhd_riemann_hlle": (None, [_P, _P, _D, _I, _D, _P, _P]),
+    "orc_mhd_advance_ct": (_I, [_I, _P, _P, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P, _P]),
+    "orc_mhd_rk2": (_I, [_I, _P, _P, _P, _P, _D, _D, _D, _D, _P]),
 }
 
 _REF_SIG = {
@@ -74,6 +81,12 @@ _REF_SIG = {
     "ref_two_body_recover_elements": (None, [_P, _D, _P]),
     "ref_minidisk_updated_conserved": (None, [_P, _P, _P, _I, _L, _L, _D, _D, _P]),
     "ref_minidisk_initial_conserved": (None, [_P, _I, _L, _L, _P]),
+    "ref_mhd_recover_primitive": (_I, [_P, _P, _D, _P]),
+    "ref_mhd_conserved_density": (None, [_P, _D, _P]),
+    "ref_mhd_flux": (None, [_P, _I, _D, _P]),
+    "ref_mhd_fastest_speed": (_D, [_P, _D]),
+    "ref_mhd_riemann_hlle": (None, [_P, _P, _D, _I, _D, _P, _P]),
+    "ref_mhd_advance_ct": (_I, [_I, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P, _P]),
 }
 
 

@@ -95,6 +95,21 @@ int orc_minidisk_mesh_stage(const double* u, double* out, const orc_minidisk_t* 
 int orc_euler2d_mesh_stage(const double* u, double* out, int nb, int n, double dt, double dl, double theta,
                            double gamma, int hllc, int nthreads);
 
+/* ---------------- ideal MHD with constrained transport (oracle_mhd.c) ---------------- */
+/* primitive (rho, v1, v2, v3, p, B1, B2, B3); conserved (D, S1, S2, S3, E) */
+int    orc_mhd_recover_primitive(const double* u, const double* b, double gamma, double* p);
+void   orc_mhd_conserved_density(const double* p, double gamma, double* u);
+void   orc_mhd_flux(const double* p, int axis, double gamma, double* f);
+double orc_mhd_fastest_speed(const double* p, double gamma);
+void   orc_mhd_riemann_hlle(const double* pl, const double* pr, double bl, int axis, double gamma, double* f, double* e);
+/* advance_mhd_ct on a periodic N^3 cube (uc [N][N][N][5], bf1 [N+1][N][N],
+ * bf2 [N][N+1][N], bf3 [N][N][N+1]); dt from the CFL condition */
+int orc_mhd_advance_ct(int N, const double* uc, const double* bf1, const double* bf2, const double* bf3,
+                       double dl, double cfl, double theta, double gamma,
+                       double* uo, double* bo1, double* bo2, double* bo3, double* dt_out);
+int orc_mhd_rk2(int N, double* uc, double* bf1, double* bf2, double* bf3, double dl, double cfl, double theta,
+                double gamma, double* dts);
+
 /* exact Riemann solution of the 1D Euler equations (Toro ch. 4) for checks */
 void orc_euler_exact_riemann(const double* pl, const double* pr, double gamma, double xi, double* p);
 

@@ -20,6 +20,7 @@
 #include "core_ndarray.hpp"
 #include "mesh_block_index.hpp"
 #include "minidisk.hpp"
+#include "scheme_mhd_v1.hpp"
 
 namespace {
 
@@ -170,6 +171,93 @@ void ref_minidisk_initial_conserved(const double* sd, int level, long bi, long b
     for (unsigned long i = 0; i < n; ++i)
         for (unsigned long j = 0; j < n; ++j)
             to_array(u(i, j), &out[(i * n + j) * 3]);
+}
+
+// ---------------------------------------------------------------------------
+// MHD: src/physics_mhd.hpp and mara::advance_mhd_ct (src/scheme_mhd_v1.cpp:265-299)
+static mhd::primitive_t mhd_prim(const double* p)
+{
+    return mhd::primitive(p[0], {p[1], p[2], p[3]}, p[4], {p[5], p[6], p[7]});
+}
+
+int ref_mhd_recover_primitive(const double* u, const double* b, double gamma, double* p)
+{
+    try {
+        auto uu = mhd::conserved_density_t{{u[0], u[1], u[2], u[3], u[4]}};
+        auto bb = mhd::magnetic_field_vector_t{b[0], b[1], b[2]};
+        to_array(mhd::recover_primitive(uu, bb, gamma), p);
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        return std::string(e.what()).find("density") != std::string::npos ? 1 : 2;
+    }
+}
+
+void ref_mhd_conserved_density(const double* p, double gamma, double* u)
+{
+    to_array(mhd::conserved_density(mhd_prim(p), gamma), u);
+}
+
+void ref_mhd_flux(const double* p, int axis, double gamma, double* f)
+{
+    to_array(mhd::flux(mhd_prim(p), geometric::unit_vector_on(axis), gamma), f);
+}
+
+double ref_mhd_fastest_speed(const double* p, double gamma)
+{
+    return mhd::fastest_speed(mhd_prim(p), gamma).value;
+}
+
+void ref_mhd_riemann_hlle(const double* pl, const double* pr, double bl, int axis, double gamma, double* f, double* e)
+{
+    auto [flux, ef] = mhd::riemann_hlle(mhd_prim(pl), mhd_prim(pr), bl, geometric::unit_vector_on(axis), gamma);
+    to_array(flux, f);
+    e[0] = ef.component_1().value;
+    e[1] = ef.component_2().value;
+    e[2] = ef.component_3().value;
+}
+
+// One call of advance_mhd_ct with the periodic boundary extension.  Arrays in
+// logical order: uc [N][N][N][5], bf1 [N+1][N][N], bf2 [N][N+1][N], bf3 [N][N][N+1].
+int ref_mhd_advance_ct(int N, const double* uc, const double* bf1, const double* bf2, const double* bf3,
+                       double dl, double cfl, double plm, double* uo, double* bo1, double* bo2, double* bo3, double* dt)
+{
+    using ulong = unsigned long;
+    const ulong n = N;
+    auto ua = nd::make_array(nd::indexing([uc, n] (auto i, auto j, auto k)
+    {
+        const double* z = &uc[((i * n + j) * n + k) * 5];
+        return mhd::conserved_density_t{{z[0], z[1], z[2], z[3], z[4]}};
+    }), nd::uivec(n, n, n)) | nd::to_shared();
+    auto face = [] (const double* src, ulong a, ulong b, ulong c)
+    {
+        return nd::make_array(nd::indexing([src, b, c] (auto i, auto j, auto k)
+        {
+            return mhd::unit_magnetic_field(src[(i * b + j) * c + k]);
+        }), nd::uivec(a, b, c)) | nd::to_shared();
+    };
+    auto b1 = face(bf1, n + 1, n, n);
+    auto b2 = face(bf2, n, n + 1, n);
+    auto b3 = face(bf3, n, n, n + 1);
+    try {
+        auto bc = mara::local_periodic_boundary_extension();
+        auto [t, u_, b1_, b2_, b3_] = mara::advance_mhd_ct(0.0, ua, b1, b2, b3, dl, cfl, plm, *bc);
+        *dt = t.value;
+        for (ulong i = 0; i < n; ++i)
+            for (ulong j = 0; j < n; ++j)
+                for (ulong k = 0; k < n; ++k) to_array(u_(i, j, k), &uo[((i * n + j) * n + k) * 5]);
+        auto out = [] (auto arr, double* dst, ulong a, ulong b, ulong c)
+        {
+            for (ulong i = 0; i < a; ++i)
+                for (ulong j = 0; j < b; ++j)
+                    for (ulong k = 0; k < c; ++k) dst[(i * b + j) * c + k] = arr(i, j, k).value;
+        };
+        out(b1_, bo1, n + 1, n, n);
+        out(b2_, bo2, n, n + 1, n);
+        out(b3_, bo3, n, n, n + 1);
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        return std::string(e.what()).find("density") != std::string::npos ? 1 : 2;
+    }
 }
 
 } // extern "C"

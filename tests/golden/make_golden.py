@@ -127,6 +127,30 @@ def minidisk_cases(rng):
     return dict(sd=np.array(sds), uc=np.array(ucs), pe=np.array(pes), out=np.array(outs), meta=np.array(meta))
 
 
+def mhd_cases(rng):
+    """mhd::riemann_hlle vectors and one advance_mhd_ct call on an 8^3 periodic cube."""
+    m = 300
+    g = 5.0 / 3.0
+    pl = np.column_stack([rng.uniform(0.2, 2, m), rng.normal(0, 1, (m, 3)), rng.uniform(0.2, 2, m), rng.normal(0, 1, (m, 3))])
+    pr = np.column_stack([rng.uniform(0.2, 2, m), rng.normal(0, 1, (m, 3)), rng.uniform(0.2, 2, m), rng.normal(0, 1, (m, 3))])
+    bl = rng.normal(0, 1, m)
+    axis = rng.integers(1, 4, m).astype(np.int32)
+    f = np.zeros((m, 5))
+    e = np.zeros((m, 3))
+    for i in range(m):
+        oracle.ref().ref_mhd_riemann_hlle(P(np.ascontiguousarray(pl[i])), P(np.ascontiguousarray(pr[i])), bl[i],
+                                          int(axis[i]), g, P(f[i]), P(e[i]))
+    from mara3p_amd.problems import orszag_tang_3d
+    N = 8
+    uc, b1, b2, b3 = orszag_tang_3d(N, seed=3, perturbation=0.05)
+    ucl = np.ascontiguousarray(np.moveaxis(uc, 0, -1))
+    uo, o1, o2, o3, dt = np.zeros_like(ucl), np.zeros_like(b1), np.zeros_like(b2), np.zeros_like(b3), np.zeros(1)
+    st = oracle.ref().ref_mhd_advance_ct(N, P(ucl), P(b1), P(b2), P(b3), 1.0 / N, 0.33, 1.5, P(uo), P(o1), P(o2), P(o3), P(dt))
+    assert st == 0
+    return dict(pl=pl, pr=pr, bl=bl, axis=axis, flux=f, efield=e, N=N, uc=ucl, bf1=b1, bf2=b2, bf3=b3,
+                uc_out=uo, bf1_out=o1, bf2_out=o2, bf3_out=o3, dt=dt)
+
+
 def main():
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libmara_ref.so is not built (make -C oracle ref)")
@@ -136,6 +160,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "iso2d_hlle.npz"), **iso_cases(rng))
     np.savez_compressed(os.path.join(HERE, "two_body.npz"), **two_body_cases(rng))
     np.savez_compressed(os.path.join(HERE, "minidisk_block.npz"), **minidisk_cases(rng))
+    np.savez_compressed(os.path.join(HERE, "mhd_ct.npz"), **mhd_cases(rng))
     print("golden vectors written to", HERE)
 
 
