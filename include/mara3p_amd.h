@@ -154,6 +154,21 @@ int  m3p_mhd_advance_ct(const double* uc, const double* bf1, const double* bf2, 
  * src/mesh_cartesian_3d.hpp:290-301) into out (device double). */
 int  m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, int N, double* out, void* stream);
 
+/* Special-relativistic hydrodynamics (pointwise, FP64) -------------------- */
+/* srhd::recover_primitive, src/physics_srhd.hpp:224-279 (Newton on the
+ * pressure, <= 50 iterations, tolerance 1e-12): u, p [5][nzones]; blocking;
+ * M3P_ERR_NO_CONVERGENCE / M3P_ERR_NEGATIVE_PRESSURE as the reference throws. */
+int m3p_srhd_recover_primitive_array(const double* u, double* p, long nzones, double gamma_law_index,
+                                     double temperature_floor, void* stream);
+/* srhd::conserved_density, :285-299: p, u [5][nzones]. */
+int m3p_srhd_conserved_density_array(const double* p, double* u, long nzones, double gamma_law_index, void* stream);
+/* srhd::riemann_solver (Mignone & Bodo 2005), :377-496, per face:
+ * pl, pr, f [5][nfaces]; mode 0 = riemann_hlle, 1 = riemann_hllc (face at
+ * rest), 2 = contact speed (written to f[0][i]), 3 = riemann_hllc through a
+ * face moving at face_speed.  Blocking. */
+int m3p_srhd_riemann_array(const double* pl, const double* pr, double* f, long nfaces, int axis,
+                           double gamma_law_index, int mode, double face_speed, void* stream);
+
 /* Tile selection and autotuning (no reference counterpart: the reference runs
  * one closure per block on a CPU thread pool).  A 2D stage workgroup owns TJ
  * contiguous columns of a block and marches down H rows; the compiled

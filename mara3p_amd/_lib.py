@@ -101,6 +101,9 @@ SIGNATURES = {
     "m3p_mhd_advance_ct": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P,
                                 _P, _L, _I, _P]),
     "m3p_mhd_max_div_b": (_I, [_P, _P, _P, _I, _P, _P]),
+    "m3p_srhd_recover_primitive_array": (_I, [_P, _P, _L, _D, _D, _P]),
+    "m3p_srhd_conserved_density_array": (_I, [_P, _P, _L, _D, _P]),
+    "m3p_srhd_riemann_array": (_I, [_P, _P, _P, _L, _I, _D, _I, _D, _P]),
 }
 
 _lib = None

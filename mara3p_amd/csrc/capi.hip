@@ -8,6 +8,7 @@
 #include "mara3p_amd.h"
 #include "elementwise.hpp"
 #include "mhd.hpp"
+#include "srhd.hpp"
 #include "stage1d.hpp"
 #include "stage2d.hpp"
 #include "two_body.hpp"
@@ -576,6 +577,48 @@ int m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, i
     hipLaunchKernelGGL(mhd_divb_kernel, dim3((N3 + 255) / 256), dim3(256), 0, s, bf1, bf2, bf3, N, slot);
     hipLaunchKernelGGL(copy_u64_as_double_kernel, dim3(1), dim3(1), 0, s, slot, out);
     return launch_status("m3p_mhd_max_div_b");
+}
+
+int m3p_srhd_recover_primitive_array(const double* u, double* p, long nzones, double gamma_law_index,
+                                     double temperature_floor, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !p || nzones <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_srhd_recover_primitive_array: bad arguments");
+    hipLaunchKernelGGL(srhd_recover_array_kernel, dim3((nzones + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       u, p, nzones, gamma_law_index, temperature_floor, ds->err);
+    if (int st = launch_status("srhd_recover_array_kernel")) return st;
+    int code = 0, block = 0, zone = 0;
+    if (int st = m3p_check_errors(stream, &code, &block, &zone)) return st;
+    if (code == M3P_ERR_NO_CONVERGENCE) return fail(code, "srhd::recover_primitive (root finder not converging)");
+    if (code == M3P_ERR_NEGATIVE_PRESSURE) return fail(code, "srhd::recover_primitive (negative pressure)");
+    return code ? fail(code, "m3p: kernel error") : M3P_OK;
+}
+
+int m3p_srhd_conserved_density_array(const double* p, double* u, long nzones, double gamma_law_index, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !p || nzones <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_srhd_conserved_density_array: bad arguments");
+    hipLaunchKernelGGL(srhd_conserved_array_kernel, dim3((nzones + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       p, u, nzones, gamma_law_index);
+    return launch_status("srhd_conserved_array_kernel");
+}
+
+int m3p_srhd_riemann_array(const double* pl, const double* pr, double* f, long nfaces, int axis,
+                           double gamma_law_index, int mode, double face_speed, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!pl || !pr || !f || nfaces <= 0 || axis < 1 || axis > 3 || mode < 0 || mode > 3)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_srhd_riemann_array: bad arguments");
+    hipLaunchKernelGGL(srhd_riemann_array_kernel, dim3((nfaces + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       pl, pr, f, nfaces, axis, gamma_law_index, mode, face_speed, ds->err);
+    if (int st = launch_status("srhd_riemann_array_kernel")) return st;
+    int code = 0, block = 0, zone = 0;
+    if (int st = m3p_check_errors(stream, &code, &block, &zone)) return st;
+    if (code) return fail(code, "srhd::riemann_solver (nan contact speed)");
+    return M3P_OK;
 }
 
 int m3p_set_stage2d_tile(int kind, int precision, int tj, int h)

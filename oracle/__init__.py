@@ -66,6 +66,10 @@ _SIG = {
     "orc_mhd_riemann_hlle": (None, [_P, _P, _D, _I, _D, _P, _P]),
     "orc_mhd_advance_ct": (_I, [_I, _P, _P, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P, _P]),
     "orc_mhd_rk2": (_I, [_I, _P, _P, _P, _P, _D, _D, _D, _D, _P]),
+    "orc_srhd_recover_primitive": (_I, [_P, _D, _D, _P]),
+    "orc_srhd_conserved_density": (None, [_P, _D, _P]),
+    "orc_srhd_flux": (None, [_P, _I, _D, _P]),
+    "orc_srhd_riemann": (_I, [_P, _P, _I, _D, _I, _D, _P]),
 }
 
 _REF_SIG = {
@@ -87,6 +91,9 @@ _REF_SIG = {
     "ref_mhd_fastest_speed": (_D, [_P, _D]),
     "ref_mhd_riemann_hlle": (None, [_P, _P, _D, _I, _D, _P, _P]),
     "ref_mhd_advance_ct": (_I, [_I, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P, _P]),
+    "ref_srhd_recover_primitive": (_I, [_P, _D, _D, _P]),
+    "ref_srhd_conserved_density": (None, [_P, _D, _P]),
+    "ref_srhd_riemann": (_I, [_P, _P, _I, _D, _I, _D, _P]),
 }
 
 

@@ -110,6 +110,14 @@ int orc_mhd_advance_ct(int N, const double* uc, const double* bf1, const double*
 int orc_mhd_rk2(int N, double* uc, double* bf1, double* bf2, double* bf3, double dl, double cfl, double theta,
                 double gamma, double* dts);
 
+/* ---------------- special-relativistic hydrodynamics (oracle_srhd.c) ---------------- */
+/* primitive (rho, u1, u2, u3, p), u = gamma beta; conserved (D, S1, S2, S3, tau) */
+int  orc_srhd_recover_primitive(const double* u, double gamma, double temperature_floor, double* p);
+void orc_srhd_conserved_density(const double* p, double gamma, double* u);
+void orc_srhd_flux(const double* p, int axis, double gamma, double* f);
+/* mode 0 HLLE, 1 HLLC, 2 contact speed (f[0]), 3 HLLC through a face moving at face_speed */
+int  orc_srhd_riemann(const double* pl, const double* pr, int axis, double gamma, int mode, double face_speed, double* f);
+
 /* exact Riemann solution of the 1D Euler equations (Toro ch. 4) for checks */
 void orc_euler_exact_riemann(const double* pl, const double* pr, double gamma, double xi, double* p);
 

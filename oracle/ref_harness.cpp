@@ -260,4 +260,41 @@ int ref_mhd_advance_ct(int N, const double* uc, const double* bf1, const double*
     }
 }
 
+// ---------------------------------------------------------------------------
+// SRHD: src/physics_srhd.hpp
+static srhd::primitive_t srhd_prim(const double* p) { return srhd::primitive(p[0], p[1], p[2], p[3], p[4]); }
+
+int ref_srhd_recover_primitive(const double* u, double gamma, double temperature_floor, double* p)
+{
+    try {
+        auto uu = srhd::conserved_density_t{{u[0], u[1], u[2], u[3], u[4]}};
+        to_array(srhd::recover_primitive(uu, gamma, temperature_floor), p);
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        return std::string(e.what()).find("converging") != std::string::npos ? 3 : 2;
+    }
+}
+
+void ref_srhd_conserved_density(const double* p, double gamma, double* u)
+{
+    to_array(srhd::conserved_density(srhd_prim(p), gamma), u);
+}
+
+int ref_srhd_riemann(const double* pl, const double* pr, int axis, double gamma, int mode, double face_speed, double* f)
+{
+    auto n = geometric::unit_vector_on(axis);
+    try {
+        switch (mode) {
+            case 0: to_array(srhd::riemann_hlle(srhd_prim(pl), srhd_prim(pr), n, gamma), f); break;
+            case 1: to_array(srhd::riemann_hllc(srhd_prim(pl), srhd_prim(pr), n, gamma), f); break;
+            case 2: f[0] = srhd::riemann_solver(srhd_prim(pl), srhd_prim(pr), n, gamma,
+                                                srhd::riemann_solver_mode_contact_speed_t()).value; break;
+            case 3: to_array(srhd::riemann_hllc(srhd_prim(pl), srhd_prim(pr), n, face_speed, gamma), f); break;
+        }
+        return 0;
+    } catch (const std::invalid_argument&) {
+        return 3;
+    }
+}
+
 } // extern "C"

@@ -151,6 +151,26 @@ def mhd_cases(rng):
                 uc_out=uo, bf1_out=o1, bf2_out=o2, bf3_out=o3, dt=dt)
 
 
+def srhd_cases(rng, m=300):
+    """srhd::recover_primitive and riemann_solver (all four modes) vectors."""
+    gam = rng.choice([4.0 / 3.0, 5.0 / 3.0], m)
+    pl = np.column_stack([rng.uniform(0.1, 3, m), rng.normal(0, 2, (m, 3)), rng.uniform(0.01, 5, m)])
+    pr = np.column_stack([rng.uniform(0.1, 3, m), rng.normal(0, 2, (m, 3)), rng.uniform(0.01, 5, m)])
+    axis = rng.integers(1, 4, m).astype(np.int32)
+    face = rng.normal(0, 0.3, m)
+    f = np.zeros((m, 4, 5))
+    st = np.zeros((m, 4), dtype=np.int32)
+    u = np.zeros((m, 5))
+    p = np.zeros((m, 5))
+    for i in range(m):
+        a, b = np.ascontiguousarray(pl[i]), np.ascontiguousarray(pr[i])
+        for mode in range(4):
+            st[i, mode] = oracle.ref().ref_srhd_riemann(P(a), P(b), int(axis[i]), gam[i], mode, face[i], P(f[i, mode]))
+        oracle.ref().ref_srhd_conserved_density(P(a), gam[i], P(u[i]))
+        oracle.ref().ref_srhd_recover_primitive(P(np.ascontiguousarray(u[i])), gam[i], 0.0, P(p[i]))
+    return dict(gamma=gam, pl=pl, pr=pr, axis=axis, face_speed=face, f=f, status=st, cons=u, recovered=p)
+
+
 def main():
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libmara_ref.so is not built (make -C oracle ref)")
@@ -161,6 +181,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "two_body.npz"), **two_body_cases(rng))
     np.savez_compressed(os.path.join(HERE, "minidisk_block.npz"), **minidisk_cases(rng))
     np.savez_compressed(os.path.join(HERE, "mhd_ct.npz"), **mhd_cases(rng))
+    np.savez_compressed(os.path.join(HERE, "srhd_pointwise.npz"), **srhd_cases(rng))
     print("golden vectors written to", HERE)
 
 
