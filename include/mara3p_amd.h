@@ -184,6 +184,42 @@ int m3p_autotune_minidisk(const double* u, double* out, const int* nbr, const in
                           const m3p_minidisk_data* data /* host */, int precision, int reps,
                           int* tj, int* h, float* ms, void* stream);
 
+/* Batched and quad-tree meshes -------------------------------------------- */
+/* 1D Euler stage over a work-list of blocks of n zones that may belong to
+ * many independent problems (an uneven ensemble): chain nbr [nblk][2] as
+ * m3p_euler1d_stage (-1 = outflow end), cell spacing per block dl_blk
+ * [nblk] (device).  One persistent launch walks the list; dt_dev (device,
+ * may be NULL) overrides dt. */
+int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, const int* nbr, const double* dl_blk,
+                             int nblk, int n, double dt, const double* dt_dev, double rk_b,
+                             double plm_theta, double gamma_law_index, int precision, void* stream);
+/* euler2d stage over the leaves of a 2:1-balanced quad-tree: guard fill
+ * across level jumps (amr::extend_block over amr::get_or_create_block,
+ * src/mesh_amr.hpp:300-393: same level copy, coarse neighbour injection,
+ * fine neighbours 2x2 averaged, all on primitives) into pe_work
+ * [nleaf][5][n+4][n+4], then the fused update with dl_leaf [nleaf]
+ * (cell_spacing per level, src/mesh_geometry.cpp:168-171).  recipes
+ * [nleaf][4][8] int (directions -x, +x, -y, +y):
+ * {kind 0 same / 1 coarse / 2 fine, leaf, child00, child10, child01,
+ *  child11, orthant_i, orthant_j}.  As the reference, no flux correction at
+ * level jumps. */
+int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
+                          const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
+                          double plm_theta, double gamma_law_index, int hllc, int precision, void* stream);
+/* CFL with a cell spacing per block: dt = cfl * min_b (dl_b / max speed). */
+int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, const double* dl_blk,
+                            double gamma_law_index, double cfl, double* dt_out, void* stream);
+
+/* hipGraph capture (no reference counterpart): wrap any sequence of
+ * asynchronous m3p calls issued on `stream` between m3p_graph_begin and
+ * m3p_graph_end; replay with m3p_graph_launch.  Time steps that change
+ * between replays must come from device memory (the dt_dev arguments, e.g.
+ * written by m3p_euler_cfl_dt inside the captured sequence). */
+int m3p_graph_begin(void* stream);
+int m3p_graph_end(void* stream, void** graph_exec /* host out */);
+int m3p_graph_launch(void* graph_exec, void* stream);
+int m3p_graph_destroy(void* graph_exec);
+
 /* Reductions --------------------------------------------------------------- */
 /* CFL time step for Euler states: dt = cfl * dl / max over zones of
  * (max(|v1|, |v2|) + c_s), written to dt_out (device double).  The reduction

@@ -1,0 +1,117 @@
+// Guard fill across quad-tree level jumps for a list of leaf blocks:
+// amr::extend_block (src/mesh_amr.hpp:360-393) over amr::get_or_create_block
+// (:300-328), writing the extended primitive block pe [nleaf][5][n+4][n+4].
+//
+// Per leaf and face direction d (-x, +x, -y, +y) a recipe row
+//   [kind, leaf, c00, c10, c01, c11, orthant_i, orthant_j]
+// says where the neighbour's zones come from:
+//   SAME   (0)  leaf `leaf` at the same level: copy
+//   COARSE (1)  parent leaf `leaf`: refine_block = upsample + split_block,
+//               zone (i, j) of the virtual neighbour is parent
+//               [(oi n + i) / 2][(oj n + j) / 2]  (injection)
+//   FINE   (2)  children leaves c00..c11: coarsen_blocks = tile_blocks +
+//               downsample, 0.25 * (((T[2i][2j] + T[2i][2j+1]) + T[2i+1][2j]) + T[2i+1][2j+1])
+// All copies are of primitives (the reference extends the primitive tree,
+// src/module_euler.cpp:316-318).  Corner guard zones are not read by the
+// Euler update and are left untouched.
+#pragma once
+#include "physics.hpp"
+
+namespace m3p {
+
+__device__ __forceinline__ void amr_prim(const double* u, long zpb, int leaf, int i, int j, int n,
+                                         const EulerParams& prm, double* p, bool& ok)
+{
+    const double* z = u + (size_t)leaf * 5 * zpb + (size_t)i * n + j;
+    double uu[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) uu[q] = z[q * zpb];
+    ok &= Euler<double>::recover(uu, prm, p);
+}
+
+// Zone (i, j) of the virtual block that sits in direction d of leaf b.
+__device__ __forceinline__ void amr_neighbour_zone(const double* u, const int* rec, int n, int i, int j,
+                                                   const EulerParams& prm, double* p, bool& ok)
+{
+    const long zpb = (long)n * n;
+    const int kind = rec[0];
+    if (kind == 0) {
+        amr_prim(u, zpb, rec[1], i, j, n, prm, p, ok);
+    } else if (kind == 1) {
+        amr_prim(u, zpb, rec[1], (rec[6] * n + i) >> 1, (rec[7] * n + j) >> 1, n, prm, p, ok);
+    } else {
+        double a[5], b[5], c[5], e[5];
+        auto tile = [&] (int I, int J, double* out) {
+            const int child = rec[2 + (I / n) + 2 * (J / n)];
+            amr_prim(u, zpb, child, I % n, J % n, n, prm, out, ok);
+        };
+        tile(2 * i, 2 * j, a);
+        tile(2 * i, 2 * j + 1, b);
+        tile(2 * i + 1, 2 * j, c);
+        tile(2 * i + 1, 2 * j + 1, e);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) p[q] = 0.25 * (a[q] + b[q] + c[q] + e[q]);
+    }
+}
+
+// One thread per extended zone (i, j) in [0, n+4)^2 of each leaf.
+__global__ void __launch_bounds__(256)
+amr_guard_fill_kernel(const double* u, double* pe, const int* recipes, int nleaf, int n, EulerParams prm, int* err)
+{
+    const int e = n + 4;
+    const long per = (long)e * e;
+    const long z = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= per * nleaf) return;
+    const int b = (int)(z / per);
+    const int r = (int)(z % per);
+    const int i = r / e - 2, j = r % e - 2;
+    const bool in_i = i >= 0 && i < n, in_j = j >= 0 && j < n;
+    if (!in_i && !in_j) return;   // corner guard zones: unused by the update
+    double p[5];
+    bool ok = true;
+    if (in_i && in_j) {
+        amr_prim(u, (long)n * n, b, i, j, n, prm, p, ok);
+    } else {
+        int d, ii = i, jj = j;
+        if (i < 0)       { d = 0; ii = i + n; }
+        else if (i >= n) { d = 1; ii = i - n; }
+        else if (j < 0)  { d = 2; jj = j + n; }
+        else             { d = 3; jj = j - n; }
+        amr_neighbour_zone(u, recipes + ((size_t)b * 4 + d) * 8, n, ii, jj, prm, p, ok);
+    }
+    double* o = pe + (size_t)b * 5 * per + (size_t)(i + 2) * e + (j + 2);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) o[q * per] = p[q];
+    if (!ok) report_error(err, 1, b, (i * n + j));
+}
+
+// CFL with a cell spacing per block: dt = cfl * min_b (dl_b / max speed in b).
+__global__ void __launch_bounds__(256)
+euler_min_dl_over_speed_kernel(const double* u, int nblk, long zpb, const double* dl_blk, EulerParams prm,
+                               unsigned long long* slot)
+{
+    __shared__ double part[4];
+    double m = 1e300;
+    const long total = (long)nblk * zpb;
+    for (long z = (long)blockIdx.x * blockDim.x + threadIdx.x; z < total; z += (long)gridDim.x * blockDim.x) {
+        const long b = z / zpb, i = z % zpb;
+        const double* base = u + b * 5 * zpb + i;
+        double uu[5] = { base[0], base[zpb], base[2 * zpb], base[3 * zpb], base[4 * zpb] };
+        m = fmin(m, dl_blk[b] / Euler<double>::max_speed(uu, prm));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, kWave));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = part[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmin(v, part[w]);
+        atomicMin(slot, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+__global__ void cfl_from_min_kernel(const unsigned long long* slot, double cfl, double* dt_out)
+{
+    *dt_out = cfl * __longlong_as_double((long long)*slot);
+}
+
+} // namespace m3p

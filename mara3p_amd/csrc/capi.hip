@@ -6,6 +6,7 @@
 #include <mutex>
 #include <string>
 #include "mara3p_amd.h"
+#include "amr.hpp"
 #include "elementwise.hpp"
 #include "mhd.hpp"
 #include "srhd.hpp"
@@ -35,6 +36,7 @@ int hip_check(hipError_t e, const char* what)
 struct DeviceState {
     int* err = nullptr;          // int[4]
     unsigned long long* red = nullptr;  // reduction scratch, kRedWords words
+    int cus = 0;                 // compute units (persistent-grid sizing)
 };
 
 constexpr int kMaxDevices = 64;
@@ -53,6 +55,7 @@ int device_state(DeviceState** out)
         if (int st = hip_check(hipMalloc(&s.err, 4 * sizeof(int)), "hipMalloc(error latch)")) return st;
         if (int st = hip_check(hipMemset(s.err, 0, 4 * sizeof(int)), "hipMemset(error latch)")) return st;
         if (int st = hip_check(hipMalloc(&s.red, kRedWords * sizeof(unsigned long long)), "hipMalloc(reduction)")) return st;
+        if (int st = hip_check(hipDeviceGetAttribute(&s.cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute")) return st;
     }
     *out = &s;
     return M3P_OK;
@@ -147,11 +150,13 @@ int autotune_stage2d(int kind, const typename S::Params& prm, const Ld& ld, cons
 }
 
 template<typename R>
-int launch_stage1d(const EulerParams& prm, const Stage1DArgs& a, hipStream_t s)
+int launch_stage1d(const EulerParams& prm, const Stage1DArgs& a, hipStream_t s, int cus)
 {
     const int n = a.n;
     auto go = [&] (auto kern, int TJ) {
-        hipLaunchKernelGGL(kern, dim3(a.nblk * (n / TJ)), dim3(TJ), 0, s, prm, a);
+        const long items = (long)a.nblk * (n / TJ);
+        const int grid = (int)std::min<long>(items, (long)std::max(cus, 1) * 16);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(TJ), 0, s, prm, a);
         return launch_status("stage1d_kernel");
     };
     switch (n) {
@@ -376,7 +381,7 @@ int m3p_euler1d_updated_conserved(const double* uc, const double* pe, double* ou
     a.u = uc; a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = ds->err;
     a.pe = pe; a.outflow_lower = outflow_lower; a.outflow_upper = outflow_upper;
     hipStream_t s = (hipStream_t)stream;
-    int st = precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s) : launch_stage1d<float>(prm, a, s);
+    int st = precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s, ds->cus) : launch_stage1d<float>(prm, a, s, ds->cus);
     if (st) return st;
     return finish_blocking(stream);
 }
@@ -459,7 +464,7 @@ int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int*
     a.u = u; a.u0 = u0; a.out = out; a.nbr = nbr; a.dt = dt; a.dl = dl; a.rk_b = rk_b;
     a.n = n; a.nblk = nblk; a.err = ds->err;
     hipStream_t s = (hipStream_t)stream;
-    return precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s) : launch_stage1d<float>(prm, a, s);
+    return precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s, ds->cus) : launch_stage1d<float>(prm, a, s, ds->cus);
 }
 
 int m3p_euler_cfl_dt(const double* u, int nblk, long zones_per_block, double gamma_law_index,
@@ -619,6 +624,93 @@ int m3p_srhd_riemann_array(const double* pl, const double* pr, double* f, long n
     if (int st = m3p_check_errors(stream, &code, &block, &zone)) return st;
     if (code) return fail(code, "srhd::riemann_solver (nan contact speed)");
     return M3P_OK;
+}
+
+int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, const int* nbr, const double* dl_blk,
+                             int nblk, int n, double dt, const double* dt_dev, double rk_b,
+                             double plm_theta, double gamma_law_index, int precision, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !out || !nbr || !dl_blk || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_euler1d_stage_blocks: bad arguments");
+    if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
+    EulerParams prm{ gamma_law_index, plm_theta, 0 };
+    Stage1DArgs a{};
+    a.u = u; a.u0 = u0; a.out = out; a.nbr = nbr; a.dt = dt; a.dt_dev = dt_dev; a.dl_blk = dl_blk; a.rk_b = rk_b;
+    a.n = n; a.nblk = nblk; a.err = ds->err;
+    hipStream_t s = (hipStream_t)stream;
+    return precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s, ds->cus) : launch_stage1d<float>(prm, a, s, ds->cus);
+}
+
+int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
+                          const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
+                          double plm_theta, double gamma_law_index, int hllc, int precision, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !out || !pe_work || !recipes || !dl_leaf || nleaf <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_amr_euler2d_stage: bad arguments");
+    if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
+    EulerParams prm{ gamma_law_index, plm_theta, hllc };
+    hipStream_t s = (hipStream_t)stream;
+    const long ext = (long)(n + 4) * (n + 4) * nleaf;
+    hipLaunchKernelGGL(amr_guard_fill_kernel, dim3((ext + 255) / 256), dim3(256), 0, s, u, pe_work, recipes, nleaf, n, prm, ds->err);
+    if (int st = launch_status("amr_guard_fill_kernel")) return st;
+    StageArgs a{};
+    a.u0 = u0; a.out = out; a.dt_dev = dt_dev; a.dt = dt; a.dl_blk = dl_leaf; a.rk_b = rk_b;
+    a.n = n; a.nblk = nleaf; a.err = ds->err;
+    if (precision == M3P_FP64)
+        return launch_stage2d<EulerScheme<double>>(M3P_KIND_EULER2D, prm, ExtendedLoader<EulerScheme<double>>{u, pe_work, n}, a, s);
+    return launch_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, ExtendedLoader<EulerScheme<float>>{u, pe_work, n}, a, s);
+}
+
+int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, const double* dl_blk,
+                            double gamma_law_index, double cfl, double* dt_out, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!u || !dl_blk || !dt_out || nblk <= 0 || zones_per_block <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad cfl arguments");
+    hipStream_t s = (hipStream_t)stream;
+    EulerParams prm{ gamma_law_index, 0.0, 0 };
+    unsigned long long* slot = ds->red + 3001;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, slot, 1, (unsigned long long)__builtin_bit_cast(long long, 1e300));
+    const long total = (long)nblk * zones_per_block;
+    int grid = (int)std::min<long>((total + 255) / 256, 2048);
+    hipLaunchKernelGGL(euler_min_dl_over_speed_kernel, dim3(grid), dim3(256), 0, s, u, nblk, zones_per_block, dl_blk, prm, slot);
+    hipLaunchKernelGGL(cfl_from_min_kernel, dim3(1), dim3(1), 0, s, slot, cfl, dt_out);
+    return launch_status("m3p_euler_cfl_dt_blocks");
+}
+
+// hipGraph capture of any sequence of asynchronous m3p calls on one stream.
+int m3p_graph_begin(void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;      // library state exists before capture starts
+    return hip_check(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+}
+
+int m3p_graph_end(void* stream, void** graph_exec)
+{
+    if (!graph_exec) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_graph_end: NULL handle");
+    hipGraph_t g = nullptr;
+    if (int st = hip_check(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture")) return st;
+    hipGraphExec_t e = nullptr;
+    int st = hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    (void)hipGraphDestroy(g);
+    if (st) return st;
+    *graph_exec = (void*)e;
+    return M3P_OK;
+}
+
+int m3p_graph_launch(void* graph_exec, void* stream)
+{
+    if (!graph_exec) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_graph_launch: NULL handle");
+    return hip_check(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+int m3p_graph_destroy(void* graph_exec)
+{
+    if (!graph_exec) return M3P_OK;
+    return hip_check(hipGraphExecDestroy((hipGraphExec_t)graph_exec), "hipGraphExecDestroy");
 }
 
 int m3p_set_stage2d_tile(int kind, int precision, int tj, int h)

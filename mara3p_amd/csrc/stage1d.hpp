@@ -17,6 +17,8 @@ struct Stage1DArgs {
     double* out;
     const int* nbr;        // [nblk][2] = {prev, next}; -1 = outflow boundary
     double dt, dl, rk_b;
+    const double* dt_dev;  // device-resident dt or nullptr
+    const double* dl_blk;  // per-block cell spacing or nullptr
     int n, nblk;
     int* err;
     // extended-array mode (per-block compatibility): pe [5][n+4] primitives
@@ -24,6 +26,9 @@ struct Stage1DArgs {
     int outflow_lower, outflow_upper;
 };
 
+// Persistent work-list form: the grid is sized to the device and each
+// workgroup walks the list of (block, tile) items with a fixed stride, so one
+// launch covers any number of blocks of any mix of problems.
 template<typename R, int TJ>
 __global__ void __launch_bounds__(TJ)
 stage1d_kernel(EulerParams prm, Stage1DArgs a)
@@ -31,75 +36,81 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
     constexpr int NC = 5;
     const int n = a.n;
     const int tiles = n / TJ;
-    const int b = blockIdx.x / tiles;
-    const int c0 = (blockIdx.x % tiles) * TJ;
+    const long items = (long)a.nblk * tiles;
     const int t = threadIdx.x;
     const R theta = R(prm.theta);
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
 
     __shared__ R P[NC][TJ + 4];   // [i] = zone c0 - 2 + i
-    __shared__ R G[NC][TJ + 2];   // [i] = slope at zone c0 - 1 + i
+    __shared__ R G[NC][TJ + 2];   // [i] = half slope at zone c0 - 1 + i
     __shared__ R F[NC][TJ + 1];   // [i] = face c0 + i (left face of zone c0 + i)
 
     bool ok = true;
-    auto load = [&] (int i, R* p) {
-        // zone index i in [-2, n+2) of block b
-        if (a.pe) {
-            int e = i;
-            if (a.outflow_lower && i < 0) e = 0;
-            else if (!a.outflow_lower && a.outflow_upper && i >= n) e = n - 1;
-            for (int q = 0; q < NC; ++q) p[q] = R(a.pe[(size_t)q * (n + 4) + e + 2]);
-            return;
-        }
-        int bb = b, z = i;
-        if (i < 0) {
-            int pb = a.nbr[2 * b];
-            if (pb < 0) z = 0; else { bb = pb; z = i + n; }
-        } else if (i >= n) {
-            int nb = a.nbr[2 * b + 1];
-            if (nb < 0) z = n - 1; else { bb = nb; z = i - n; }
-        }
-        const double* src = a.u + (size_t)bb * NC * n + z;
-        double uu[NC];
-        for (int q = 0; q < NC; ++q) uu[q] = src[(size_t)q * n];
-        ok &= Euler<R>::recover(uu, prm, p);
-    };
+    for (long item = blockIdx.x; item < items; item += gridDim.x) {
+        const int b = (int)(item / tiles);
+        const int c0 = (int)(item % tiles) * TJ;
+        const double dl = a.dl_blk ? a.dl_blk[b] : a.dl;
+        auto load = [&] (int i, R* p) {
+            // zone index i in [-2, n+2) of block b
+            if (a.pe) {
+                int e = i;
+                if (a.outflow_lower && i < 0) e = 0;
+                else if (!a.outflow_lower && a.outflow_upper && i >= n) e = n - 1;
+                for (int q = 0; q < NC; ++q) p[q] = R(a.pe[(size_t)q * (n + 4) + e + 2]);
+                return;
+            }
+            int bb = b, z = i;
+            if (i < 0) {
+                int pb = a.nbr[2 * b];
+                if (pb < 0) z = 0; else { bb = pb; z = i + n; }
+            } else if (i >= n) {
+                int nb = a.nbr[2 * b + 1];
+                if (nb < 0) z = n - 1; else { bb = nb; z = i - n; }
+            }
+            const double* src = a.u + (size_t)bb * NC * n + z;
+            double uu[NC];
+            for (int q = 0; q < NC; ++q) uu[q] = src[(size_t)q * n];
+            ok &= Euler<R>::recover(uu, prm, p);
+        };
 
-    R p[NC];
-    load(c0 + t, p);
-    for (int q = 0; q < NC; ++q) P[q][t + 2] = p[q];
-    if (t < 4) {
-        int i = t < 2 ? c0 - 2 + t : c0 + TJ + (t - 2);
-        load(i, p);
-        int slot = t < 2 ? t : TJ + 2 + (t - 2);
-        for (int q = 0; q < NC; ++q) P[q][slot] = p[q];
-    }
-    __syncthreads();
-    for (int i = t; i < TJ + 2; i += TJ)
-        for (int q = 0; q < NC; ++q) G[q][i] = plm(P[q][i], P[q][i + 1], P[q][i + 2], theta);
-    __syncthreads();
-    for (int i = t; i < TJ + 1; i += TJ) {
-        R pl[NC], pr[NC], f[NC];
+        R p[NC];
+        load(c0 + t, p);
+        for (int q = 0; q < NC; ++q) P[q][t + 2] = p[q];
+        if (t < 4) {
+            int i = t < 2 ? c0 - 2 + t : c0 + TJ + (t - 2);
+            load(i, p);
+            int slot = t < 2 ? t : TJ + 2 + (t - 2);
+            for (int q = 0; q < NC; ++q) P[q][slot] = p[q];
+        }
+        __syncthreads();
+        for (int i = t; i < TJ + 2; i += TJ)
+            for (int q = 0; q < NC; ++q) G[q][i] = plm_half(P[q][i], P[q][i + 1], P[q][i + 2], theta);
+        __syncthreads();
+        for (int i = t; i < TJ + 1; i += TJ) {
+            R pl[NC], pr[NC], f[NC];
+            for (int q = 0; q < NC; ++q) {
+                pl[q] = P[q][i + 1] + G[q][i];
+                pr[q] = P[q][i + 2] - G[q][i + 1];
+            }
+            Euler<R>::template riemann<1>(pl, pr, prm, f);
+            for (int q = 0; q < NC; ++q) F[q][i] = f[q];
+        }
+        __syncthreads();
+
+        const int z = c0 + t;
+        const double* uc = a.u + (size_t)b * NC * n + z;
+        double* o = a.out + (size_t)b * NC * n + z;
         for (int q = 0; q < NC; ++q) {
-            pl[q] = P[q][i + 1] + R(0.5) * G[q][i];
-            pr[q] = P[q][i + 2] - R(0.5) * G[q][i + 1];
+            double dfx = double(F[q][t + 1]) - double(F[q][t]);
+            double res;
+            if constexpr (is_fp64<R>::value) res = uc[(size_t)q * n] - dfx * dt / dl;
+            else res = uc[(size_t)q * n] - dfx * (dt / dl);
+            if (a.u0) res = a.u0[(size_t)b * NC * n + (size_t)q * n + z] * a.rk_b + res * (1.0 - a.rk_b);
+            o[(size_t)q * n] = res;
         }
-        Euler<R>::template riemann<1>(pl, pr, prm, f);
-        for (int q = 0; q < NC; ++q) F[q][i] = f[q];
+        if (!ok) { report_error(a.err, 1, b, z); ok = true; }
+        __syncthreads();   // LDS is reused by the next item
     }
-    __syncthreads();
-
-    const int z = c0 + t;
-    const double* uc = a.u + (size_t)b * NC * n + z;
-    double* o = a.out + (size_t)b * NC * n + z;
-    for (int q = 0; q < NC; ++q) {
-        double dfx = double(F[q][t + 1]) - double(F[q][t]);
-        double res;
-        if constexpr (is_fp64<R>::value) res = uc[(size_t)q * n] - dfx * a.dt / a.dl;
-        else res = uc[(size_t)q * n] - dfx * (a.dt / a.dl);
-        if (a.u0) res = a.u0[(size_t)b * NC * n + (size_t)q * n + z] * a.rk_b + res * (1.0 - a.rk_b);
-        o[(size_t)q * n] = res;
-    }
-    if (!ok) report_error(a.err, 1, b, z);
 }
 
 } // namespace m3p

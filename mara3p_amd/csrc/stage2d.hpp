@@ -85,17 +85,20 @@ struct ExtendedLoader {
     const double* pe;
     int n;
 
-    __device__ __forceinline__ void cons(int, int r, int c, double* uu) const
+    // Block b of a list of blocks: uc + b NC n^2, pe + b NC (n+4)^2.
+    __device__ __forceinline__ void cons(int b, int r, int c, double* uu) const
     {
+        const double* base = uc + (size_t)b * S::NC * n * n;
 #pragma unroll
-        for (int q = 0; q < S::NC; ++q) uu[q] = uc[((size_t)q * n + r) * n + c];
+        for (int q = 0; q < S::NC; ++q) uu[q] = base[((size_t)q * n + r) * n + c];
     }
-    __device__ __forceinline__ bool prim(int, int r, int c, const typename S::Params&,
+    __device__ __forceinline__ bool prim(int b, int r, int c, const typename S::Params&,
                                          typename S::Real* p) const
     {
         const int e = n + 4;
+        const double* base = pe + (size_t)b * S::NC * e * e;
 #pragma unroll
-        for (int q = 0; q < S::NC; ++q) p[q] = typename S::Real(pe[((size_t)q * e + (r + 2)) * e + (c + 2)]);
+        for (int q = 0; q < S::NC; ++q) p[q] = typename S::Real(base[((size_t)q * e + (r + 2)) * e + (c + 2)]);
         return true;
     }
     __device__ __forceinline__ void cons_row(int b, int r, int c, double* uu) const { cons(b, r, c, uu); }
@@ -121,6 +124,7 @@ struct StageArgs {
     const double* dt_dev;  // device-resident dt (graph replay) or nullptr
     double dt;
     double dl;
+    const double* dl_blk;  // per-block cell spacing (quad-tree levels) or nullptr for dl
     double rk_b;           // out = u0 * rk_b + L(u) * (1 - rk_b) when u0 != nullptr
     const int* bcoord;     // [nblk][2] block coordinates, or nullptr (single block at bi0,bj0)
     int bi0, bj0;
@@ -156,6 +160,7 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
     const int c = tc.c0 + t;
     const int b = tc.b;
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const double dl = a.dl_blk ? a.dl_blk[b] : a.dl;
 
     __shared__ typename S::template TileData<H, TJ> td;
     __shared__ R hraw[7][NC][H];     // pre-pass primitives at 7 edge columns
@@ -277,7 +282,7 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
         } else {
             ld.cons_row(b, k, c, uu);
         }
-        S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, a.dl, res);
+        S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, dl, res);
         double* o = a.out + ((size_t)b * NC * n + k) * n + c;
         if (a.u0) {
             const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;

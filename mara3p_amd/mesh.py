@@ -68,3 +68,41 @@ def to_device_layout(u_aos):
 def to_reference_layout(u_soa):
     """Device SoA [nblk, nc, n(, n)] -> reference logical layout [nblk, n(, n), nc]."""
     return np.ascontiguousarray(np.moveaxis(u_soa, 1, -1))
+
+
+class EnsembleMesh1D:
+    """Many independent 1D problems in one block list.  Problem k has
+    ``lengths[k]`` zones (a multiple of the block size ``n``) on [0, 1], so its
+    cell spacing is 1 / lengths[k]; its blocks are chained with outflow ends
+    (src/mara.cpp:123-143).  Blocks are stored problem after problem."""
+
+    def __init__(self, lengths, block_size):
+        self.n = int(block_size)
+        self.lengths = np.asarray(lengths, dtype=np.int64)
+        if np.any(self.lengths % self.n) or np.any(self.lengths <= 0):
+            raise ValueError("every problem length must be a positive multiple of the block size")
+        self.blocks_per_problem = self.lengths // self.n
+        self.first_block = np.concatenate([[0], np.cumsum(self.blocks_per_problem)[:-1]])
+        self.nblk = int(self.blocks_per_problem.sum())
+
+    def neighbours(self):
+        nbr = np.empty((self.nblk, 2), dtype=np.int32)
+        for f, m in zip(self.first_block, self.blocks_per_problem):
+            b = np.arange(f, f + m)
+            nbr[f:f + m, 0] = b - 1
+            nbr[f:f + m, 1] = b + 1
+            nbr[f, 0] = -1
+            if m > 1:
+                nbr[f + m - 1, 1] = -1
+            else:
+                nbr[f, 1] = f    # one block: lower outflow only, as a 1-block chain
+        return nbr
+
+    def cell_spacing(self):
+        return np.repeat(1.0 / self.lengths.astype(np.float64), self.blocks_per_problem)
+
+    def problem_slice(self, k):
+        return slice(int(self.first_block[k]), int(self.first_block[k] + self.blocks_per_problem[k]))
+
+    def zones(self):
+        return int(self.lengths.sum())

@@ -146,6 +146,53 @@ def sod(nblk=16, block_size=64, gamma=1.4):
     return np.ascontiguousarray(u.reshape(nblk, n, 5).transpose(0, 2, 1))
 
 
+def sedov_amr(tree, block_size, energy=1.0, radius=0.01, rho=1.0, p=1e-5, gamma=5.0 / 3.0):
+    """2D Sedov blast on a quad-tree of leaves over [-1/2, 1/2]^2: rho = 1,
+    v = 0, p = 1e-5, plus total energy `energy` deposited uniformly (per unit
+    area) in the zones whose centres lie within `radius` of the origin.
+    Returns conserved [nleaf, 5, n, n]."""
+    n = block_size
+    lev = tree.levels().astype(np.float64)
+    ij = tree.coordinates().astype(np.float64)
+    k = (np.arange(n) + 0.5)[None, :]
+    dl = 1.0 / (n * 2.0 ** lev)
+    xc = -0.5 + ij[:, 0, None] / 2.0 ** lev[:, None] + k * dl[:, None]
+    yc = -0.5 + ij[:, 1, None] / 2.0 ** lev[:, None] + k * dl[:, None]
+    x = np.broadcast_to(xc[:, :, None], (len(lev), n, n))
+    y = np.broadcast_to(yc[:, None, :], (len(lev), n, n))
+    inside = x * x + y * y < radius * radius
+    area = np.broadcast_to((dl * dl)[:, None, None], inside.shape)
+    e_density = energy / float(np.sum(area[inside]))
+    z = np.zeros(inside.shape)
+    pres = np.where(inside, (gamma - 1.0) * e_density, p)
+    return np.ascontiguousarray(euler_conserved(np.full(inside.shape, rho), z, z, z, pres, gamma))
+
+
+def riemann_ensemble(count=512, block_size=64, lmin=512, lmax=8192, seed=1, gamma=1.4):
+    """An ensemble of `count` 1D Riemann problems on [0, 1], each split at its
+    centre.  Lengths are drawn uniformly from the multiples of `block_size`
+    in [lmin, lmax]; left / right rho and p are log-uniform in [0.1, 10];
+    v = 0.  Returns (EnsembleMesh1D, conserved [nblk, 5, n])."""
+    from .mesh import EnsembleMesh1D
+    n = block_size
+    k = np.arange(count)
+    choices = (lmax - lmin) // n + 1
+    lengths = lmin + n * np.minimum((uniform01(seed, k, 0) * choices).astype(np.int64), choices - 1)
+    mesh = EnsembleMesh1D(lengths, n)
+    logu = lambda s: 10.0 ** (2.0 * uniform01(seed, k, s) - 1.0)
+    rl, rr, pl, pr = logu(1), logu(2), logu(3), logu(4)
+    u = np.empty((mesh.nblk, 5, n))
+    for q in range(count):
+        L = int(lengths[q])
+        left = (np.arange(L) + 0.5) / L < 0.5
+        rho = np.where(left, rl[q], rr[q])
+        p = np.where(left, pl[q], pr[q])
+        z = np.zeros(L)
+        uq = euler_conserved(rho, z, z, z, p, gamma)          # [L, 5]
+        u[mesh.problem_slice(q)] = uq.reshape(L // n, n, 5).transpose(0, 2, 1)
+    return mesh, u
+
+
 def orszag_tang_3d(N, seed=1, perturbation=1e-3, gamma=5.0 / 3.0):
     """3D Orszag-Tang-style state on the periodic unit cube with N^3 cells:
     rho = 25/(36 pi), p = 5/(12 pi), v = (-sin 2 pi y, sin 2 pi x, vz),

@@ -12,7 +12,8 @@ import torch
 
 from . import _lib
 from ._lib import check, lib, MinidiskData, PRECISIONS
-from .mesh import ChainMesh1D, UniformMesh2D
+from .amr import QuadTree
+from .mesh import ChainMesh1D, EnsembleMesh1D, UniformMesh2D
 
 
 def _ptr(t):
@@ -153,6 +154,137 @@ class Euler1DSolver(_StateSolver):
         self.stage(self.u, None, self.u1, dt)
         self.stage(self.u1, self.u, self.u2, dt)
         self.u, self.u2 = self.u2, self.u
+
+
+class AmrEuler2DSolver(_StateSolver):
+    """euler2d on the leaves of a 2:1-balanced quad-tree (module_euler.cpp:263-323
+    over amr::extend_block, src/mesh_amr.hpp:300-393).  All leaves advance with
+    one dt; each uses its own level's cell spacing.  One guard-fill launch and
+    one fused stage launch per RK stage."""
+    nc = 5
+
+    def __init__(self, tree: QuadTree, block_size, gamma=5.0 / 3.0, plm_theta=1.5, precision="fp64",
+                 hllc=False, domain_size=1.0, device="cuda"):
+        self.tree = tree
+        self.n = int(block_size)
+        self.nleaf = len(tree.leaves)
+        self.gamma, self.theta, self.hllc = gamma, plm_theta, int(hllc)
+        self.precision = PRECISIONS[precision]
+        n = self.n
+        self._alloc((self.nleaf, 5, n, n), device)
+        self.pe = torch.zeros((self.nleaf, 5, n + 4, n + 4), dtype=torch.float64, device=self.device)
+        self.recipes = torch.as_tensor(tree.recipes(), device=self.device)
+        self.dl_host = domain_size / float(n) / (2.0 ** tree.levels().astype(np.float64))
+        self.dl = torch.as_tensor(self.dl_host, dtype=torch.float64, device=self.device)
+        self.dt_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
+
+    def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None):
+        check(lib().m3p_amr_euler2d_stage(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.pe), _ptr(self.recipes),
+                                          _ptr(self.dl), self.nleaf, self.n, dt, _ptr(dt_dev), rk_b, self.theta,
+                                          self.gamma, self.hllc, self.precision, _stream()))
+
+    def step(self, dt):
+        self.stage(self.u, None, self.u1, dt)
+        self.stage(self.u1, self.u, self.u2, dt)
+        self.u, self.u2 = self.u2, self.u
+
+    def cfl_dt(self, cfl):
+        check(lib().m3p_euler_cfl_dt_blocks(_ptr(self.u), self.nleaf, self.n * self.n, _ptr(self.dl), self.gamma,
+                                            cfl, _ptr(self.dt_dev), _stream()))
+        return self.dt_dev
+
+    def step_cfl(self, cfl):
+        dt_dev = self.cfl_dt(cfl)
+        self.stage(self.u, None, self.u1, 0.0, dt_dev=dt_dev)
+        self.stage(self.u1, self.u, self.u2, 0.0, dt_dev=dt_dev)
+        self.u, self.u2 = self.u2, self.u
+
+    def conserved_totals_weighted(self):
+        """Totals of U * dA over the leaves (host, fsum)."""
+        u = self.state()
+        import math
+        w = (self.dl_host ** 2)[:, None, None, None] * u
+        return np.array([math.fsum(w[:, q].ravel()) for q in range(5)])
+
+
+class Euler1DEnsembleSolver(_StateSolver):
+    """Many independent euler1d problems (outflow ends, cell spacing 1/L each)
+    advanced together by one persistent work-list launch per RK stage."""
+    nc = 5
+
+    def __init__(self, mesh: EnsembleMesh1D, gamma=1.4, plm_theta=1.5, precision="fp64", device="cuda"):
+        self.mesh = mesh
+        self.gamma, self.theta = gamma, plm_theta
+        self.precision = PRECISIONS[precision]
+        self._alloc((mesh.nblk, 5, mesh.n), device)
+        self.nbr = torch.as_tensor(mesh.neighbours(), device=self.device)
+        self.dl = torch.as_tensor(mesh.cell_spacing(), dtype=torch.float64, device=self.device)
+        self.dt_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
+
+    def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None):
+        check(lib().m3p_euler1d_stage_blocks(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), _ptr(self.dl),
+                                             self.mesh.nblk, self.mesh.n, dt, _ptr(dt_dev), rk_b, self.theta,
+                                             self.gamma, self.precision, _stream()))
+
+    def step(self, dt):
+        self.stage(self.u, None, self.u1, dt)
+        self.stage(self.u1, self.u, self.u2, dt)
+        self.u, self.u2 = self.u2, self.u
+
+    def cfl_dt(self, cfl):
+        check(lib().m3p_euler_cfl_dt_blocks(_ptr(self.u), self.mesh.nblk, self.mesh.n, _ptr(self.dl), self.gamma,
+                                            cfl, _ptr(self.dt_dev), _stream()))
+        return self.dt_dev
+
+    def step_cfl(self, cfl):
+        dt_dev = self.cfl_dt(cfl)
+        self.stage(self.u, None, self.u1, 0.0, dt_dev=dt_dev)
+        self.stage(self.u1, self.u, self.u2, 0.0, dt_dev=dt_dev)
+        self.u, self.u2 = self.u2, self.u
+
+
+class Graph:
+    """A hipGraph of asynchronous library calls issued on the current stream.
+
+        with Graph() as g:
+            solver.step_pair_cfl(cfl)      # any m3p calls
+        g.replay()
+
+    Captured calls must take changing time steps from device memory."""
+
+    def __init__(self):
+        self.handle = ctypes.c_void_p(0)
+
+    def __enter__(self):
+        check(lib().m3p_graph_begin(_stream()))
+        return self
+
+    def __exit__(self, kind, value, tb):
+        st = lib().m3p_graph_end(_stream(), ctypes.byref(self.handle))
+        if kind is None:
+            check(st)
+        return False
+
+    def replay(self):
+        check(lib().m3p_graph_launch(self.handle, _stream()))
+
+    def __del__(self):
+        if self.handle and self.handle.value:
+            lib().m3p_graph_destroy(self.handle)
+            self.handle = ctypes.c_void_p(0)
+
+
+def cfl_step_pair(solver, cfl):
+    """Two CFL-limited RK2 steps that end in the buffer they began in
+    (u -> u1 -> u2, then u2 -> u1 -> u): the unit a Graph captures, since a
+    graph replays fixed buffer addresses."""
+    u, u1, u2 = solver.u, solver.u1, solver.u2
+    for a, b in ((u, u2), (u2, u)):
+        solver.u = a
+        dt = solver.cfl_dt(cfl)
+        solver.stage(a, None, u1, 0.0, dt_dev=dt)
+        solver.stage(u1, a, b, 0.0, dt_dev=dt)
+    solver.u, solver.u1, solver.u2 = u, u1, u2
 
 
 class MhdSolver:

@@ -1,0 +1,297 @@
+"""Quad-tree meshes with level jumps: topology, the oracle's guard fill
+(amr::get_or_create_block / extend_block, src/mesh_amr.hpp:300-393), and the
+HIP guard-fill + euler2d stage against that oracle."""
+import numpy as np
+import pytest
+
+from oracle import P, orc
+from oracle import amr as oamr
+from mara3p_amd import amr
+from mara3p_amd.amr import QuadTree
+
+G = 5.0 / 3.0
+
+
+def offset_tree(depth=5):
+    """A tree refined towards an off-centre point, so that leaves span several
+    levels and the level jumps are not mirror-symmetric."""
+    def refine(b):
+        l, i, j = b
+        if l == 0:
+            return True
+        s = float(1 << l)
+        cx, cy = -0.5 + (i + 0.5) / s, -0.5 + (j + 0.5) / s
+        return np.hypot(cx - 0.13, cy + 0.21) < 0.45 / l
+    return QuadTree.from_predicate(refine, depth)
+
+
+def finest_level_map(tree):
+    L = max(tree.levels())
+    m = np.full((1 << L, 1 << L), -1)
+    for l, i, j in tree.leaves:
+        s = L - l
+        m[i << s:(i + 1) << s, j << s:(j + 1) << s] = l
+    return m
+
+
+# --------------------------------------------------------------------------- topology
+@pytest.mark.parametrize("make", [lambda: amr.nested_tree(6, 1.5), offset_tree, lambda: amr.nested_tree(4, 0.5)])
+def test_tree_is_2to1_balanced_and_covers(make):
+    t = make()
+    L = max(t.levels())
+    areas = sum(4.0 ** (L - l) for l, _, _ in t.leaves)
+    assert areas == 4.0 ** L                     # leaves tile the square exactly once
+    m = finest_level_map(t)
+    assert np.all(m >= 0)
+    for ax in (0, 1):
+        assert np.abs(m - np.roll(m, 1, axis=ax)).max() <= 1
+
+
+def test_reference_balance_rule():
+    """amr::valid_mesh_tree (src/mesh_amr.hpp:417-449) splits a leaf whose
+    same-level edge neighbour is a node of depth > 1; after balancing no leaf
+    has one, and balancing is idempotent."""
+    t = offset_tree()
+    leaves = set(t.leaves)
+
+    def node_depth(b):
+        l, i, j = b
+        d = [L - l for L, a, c in leaves if L > l and (a >> (L - l), c >> (L - l)) == (i, j)]
+        return max(d) if d else 0
+    for b in t.leaves:
+        for axis, side in amr.DIRECTIONS:
+            nb = amr.neighbour_index(b, axis, side)
+            assert node_depth(nb) <= 1
+    assert t.balanced().leaves == t.leaves
+
+
+def test_nested_sedov_tree_size():
+    t = amr.nested_tree(6, 1.5)
+    assert len(t.leaves) == 1408
+    assert sorted(set(t.levels().tolist())) == [3, 4, 5, 6]
+    kinds = np.bincount(t.recipes()[:, :, 0].ravel(), minlength=3)
+    assert kinds[amr.COARSE] > 0 and kinds[amr.FINE] > 0
+
+
+def test_recipes_name_the_right_blocks():
+    t = offset_tree()
+    rec = t.recipes()
+    for k, b in enumerate(t.leaves):
+        for d, (axis, side) in enumerate(amr.DIRECTIONS):
+            nb = amr.neighbour_index(b, axis, side)
+            r = rec[k, d]
+            if r[0] == amr.SAME:
+                assert t.leaves[r[1]] == nb
+            elif r[0] == amr.COARSE:
+                assert t.leaves[r[1]] == (nb[0] - 1, nb[1] >> 1, nb[2] >> 1)
+                assert (r[6], r[7]) == (nb[1] & 1, nb[2] & 1)
+            else:
+                kids = [t.leaves[c] for c in r[2:6]]
+                l, i, j = nb
+                assert kids == [(l + 1, 2 * i, 2 * j), (l + 1, 2 * i + 1, 2 * j),
+                                (l + 1, 2 * i, 2 * j + 1), (l + 1, 2 * i + 1, 2 * j + 1)]
+
+
+# --------------------------------------------------------------------------- oracle pins
+def _finest_image(tree, prim, n, L):
+    """The leaves' data as one piecewise-constant image at the finest level."""
+    N = n << L
+    img = np.empty((N, N) + prim.shape[3:])
+    for k, (l, i, j) in enumerate(tree.leaves):
+        s = 1 << (L - l)
+        img[i * n * s:(i + 1) * n * s, j * n * s:(j + 1) * n * s] = np.repeat(np.repeat(prim[k], s, 0), s, 1)
+    return img
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_oracle_guard_fill_is_image_area_average(n):
+    """Injection from a coarse neighbour and 2x2 means of fine ones are the
+    area average of the piecewise-constant image of the leaves over each
+    guard zone.  Integer data keeps every average exact, so the comparison
+    is ==; corner guard zones stay NaN (never read)."""
+    t = offset_tree()
+    L = max(t.levels())
+    rng = np.random.default_rng(5)
+    prim = rng.integers(0, 1000, size=(len(t.leaves), n, n, 2)).astype(np.float64)
+    pe = oamr.extend_leaves(prim, t.leaves, n)
+    img = _finest_image(t, prim, n, L)
+    N = n << L
+    for k, (l, i, j) in enumerate(t.leaves):
+        s = 1 << (L - l)
+        for a in range(-2, n + 2):
+            for b in range(-2, n + 2):
+                inside_a, inside_b = 0 <= a < n, 0 <= b < n
+                if not inside_a and not inside_b:
+                    assert np.all(np.isnan(pe[k, a + 2, b + 2]))
+                    continue
+                ga, gb = (i * n + a) % (n << l), (j * n + b) % (n << l)
+                box = img[ga * s:(ga + 1) * s, gb * s:(gb + 1) * s].reshape(-1, 2)
+                assert np.array_equal(pe[k, a + 2, b + 2], box.sum(axis=0) / (s * s)), (k, a, b)
+        assert N == img.shape[0]
+
+
+def _kh_leaves(tree, n, seed=3):
+    """A smooth KH-like state sampled at the leaves' zone centres, reference layout [nleaf][n][n][5]."""
+    lev = tree.levels()
+    ij = tree.coordinates()
+    out = np.empty((len(tree.leaves), n, n, 5))
+    k = np.arange(n) + 0.5
+    for q, (l, (i, j)) in enumerate(zip(lev, ij)):
+        dl = 1.0 / (n << int(l))
+        x = -0.5 + i / float(1 << l) + k[:, None] * dl + 0 * k[None, :]
+        y = -0.5 + j / float(1 << l) + k[None, :] * dl + 0 * k[:, None]
+        rho = 1.0 + 0.5 * np.tanh(np.sin(2 * np.pi * y) * 4) + 0.1 * np.cos(2 * np.pi * (x + 2 * y))
+        vx = 0.3 * np.sin(2 * np.pi * y)
+        vy = 0.1 * np.sin(2 * np.pi * x + seed)
+        p = 1.0 + 0.2 * np.cos(2 * np.pi * x)
+        ke = 0.5 * rho * (vx * vx + vy * vy)
+        out[q] = np.stack([rho, rho * vx, rho * vy, 0 * rho, ke + p / (G - 1.0)], axis=-1)
+    return out
+
+
+def test_oracle_stage_without_level_jumps_equals_uniform_mesh():
+    """On a uniform tree the quad-tree stage equals the uniform-mesh oracle
+    stage (periodic next_on / prev_on guard fill, one cell spacing) exactly."""
+    depth, n = 2, 8
+    t = QuadTree.from_predicate(lambda b: True, depth)
+    u = _kh_leaves(t, n)
+    got = oamr.stage(u, t.leaves, t.levels(), n, 1e-3, 1.5, G, 0)
+    # uniform mesh block b = i * nb + j, reference layout [nb*nb][n][n][5]
+    nb = 1 << depth
+    order = [t.index[(depth, b // nb, b % nb)] for b in range(nb * nb)]
+    um = np.ascontiguousarray(u[order])
+    out = np.empty_like(um)
+    assert orc().orc_euler2d_mesh_stage(P(um), P(out), nb, n, 1e-3, 1.0 / (n * nb), 1.5, G, 0, 1) == 0
+    assert np.array_equal(got[order], out)
+
+
+def test_oracle_stage_keeps_uniform_state_across_level_jumps():
+    t = offset_tree(4)
+    n = 4
+    prim = np.array([1.3, 0.0, 0.0, 0.0, 0.7])
+    u1 = np.zeros(5)
+    orc().orc_euler_conserved_density(P(prim), G, P(u1))
+    u = np.broadcast_to(u1, (len(t.leaves), n, n, 5)).copy()
+    got = oamr.stage(u, t.leaves, t.levels(), n, 1e-3, 1.5, G, 0)
+    assert np.array_equal(got, u)
+
+
+# --------------------------------------------------------------------------- GPU
+torch = pytest.importorskip("torch")
+gpu = pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a HIP device")
+
+
+def _to_dev_layout(u_ref):
+    return np.ascontiguousarray(np.moveaxis(u_ref, -1, 1))
+
+
+def _to_ref_layout(u_dev):
+    return np.ascontiguousarray(np.moveaxis(u_dev, 1, -1))
+
+
+@pytest.mark.gpu
+@gpu
+@pytest.mark.parametrize("n,hllc,depth", [(16, 0, 5), (16, 1, 5), (32, 0, 4)])
+def test_gpu_amr_stage_fp64_matches_oracle(n, hllc, depth):
+    from mara3p_amd.solvers import AmrEuler2DSolver
+    t = offset_tree(depth)
+    u = _kh_leaves(t, n)
+    s = AmrEuler2DSolver(t, n, gamma=G, hllc=bool(hllc))
+    s.set_state(_to_dev_layout(u))
+    dt = 2e-4
+    s.step(dt)
+    s.step(dt)
+    s.check_errors()
+    ref = oamr.rk2(u, t.leaves, t.levels(), n, dt, 1.5, G, hllc, steps=2)
+    got = _to_ref_layout(s.state())
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.gpu
+@gpu
+def test_gpu_amr_guard_fill_matches_oracle():
+    """The extended primitive blocks written by the guard-fill kernel equal the
+    oracle's whole-block assembly on every non-corner zone."""
+    from mara3p_amd.solvers import AmrEuler2DSolver
+    n = 16
+    t = offset_tree()
+    u = _kh_leaves(t, n)
+    s = AmrEuler2DSolver(t, n, gamma=G)
+    s.set_state(_to_dev_layout(u))
+    s.stage(s.u, None, s.u1, 1e-4)
+    torch.cuda.synchronize()
+    pe_gpu = np.moveaxis(s.pe.cpu().numpy(), 1, -1)
+    pe_ref = oamr.extend_leaves(oamr.recover_primitives(u, G), t.leaves, n)
+    mask = ~np.isnan(pe_ref)
+    assert np.array_equal(pe_gpu[mask], pe_ref[mask])
+
+
+@pytest.mark.gpu
+@gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_sedov_amr_mirror_symmetry(precision):
+    """The nested tree and the Sedov state are symmetric under x -> -x; so is
+    the solution after CFL-limited steps through every kind of level jump."""
+    from mara3p_amd.problems import sedov_amr
+    from mara3p_amd.solvers import AmrEuler2DSolver
+    t = amr.nested_tree(5, 1.5)
+    n = 16
+    s = AmrEuler2DSolver(t, n, precision=precision)
+    s.set_state(sedov_amr(t, n))
+    for _ in range(10):
+        s.step_cfl(0.4)
+    s.check_errors()
+    u = s.state()
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    for k, (l, i, j) in enumerate(t.leaves):
+        m = t.index[(l, (1 << l) - 1 - i, j)]
+        a, b = u[k, 0], u[m, 0][::-1, :]
+        assert np.abs(a - b).max() <= tol * np.abs(a).max()
+
+
+@pytest.mark.gpu
+@gpu
+def test_gpu_amr_cfl_dt_blocks():
+    from mara3p_amd.problems import sedov_amr
+    from mara3p_amd.solvers import AmrEuler2DSolver
+    t = amr.nested_tree(4, 1.5)
+    n = 16
+    s = AmrEuler2DSolver(t, n)
+    u = sedov_amr(t, n)
+    s.set_state(u)
+    dt = float(s.cfl_dt(0.4).item())
+    D = u[:, 0]
+    v1, v2, v3 = u[:, 1] / D, u[:, 2] / D, u[:, 3] / D
+    pg = (u[:, 4] - 0.5 * (v1 * u[:, 1] + v2 * u[:, 2] + v3 * u[:, 3])) * (G - 1.0)
+    a = np.maximum(np.abs(v1), np.abs(v2)) + np.sqrt(G * pg / D)
+    want = 0.4 * np.min(s.dl_host[:, None, None] / a)
+    assert abs(dt - want) <= 1e-14 * want
+
+
+@pytest.mark.gpu
+@gpu
+def test_gpu_graph_replay_matches_eager():
+    """A hipGraph of two CFL-limited RK2 steps (guard fill, stage kernels and
+    the device-side CFL reduction) replayed three times, with eager work in
+    between, equals the same calls made eagerly, exactly."""
+    from mara3p_amd.problems import sedov_amr
+    from mara3p_amd.solvers import AmrEuler2DSolver, Graph, cfl_step_pair
+    t = amr.nested_tree(4, 1.5)
+    n = 16
+    u0 = sedov_amr(t, n)
+    eager, graphed = AmrEuler2DSolver(t, n), AmrEuler2DSolver(t, n)
+    eager.set_state(u0)
+    graphed.set_state(u0)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        with Graph() as g:
+            cfl_step_pair(graphed, 0.4)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        with torch.cuda.stream(stream):
+            g.replay()
+        cfl_step_pair(eager, 0.4)
+        torch.cuda.synchronize()
+        _ = (eager.u * 2.0).sum().item()        # unrelated eager work between replays
+        assert torch.equal(graphed.u, eager.u)
+        assert torch.equal(graphed.dt_dev, eager.dt_dev)
