@@ -193,12 +193,15 @@ int m3p_autotune_minidisk(const double* u, double* out, const int* nbr, const in
 int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, const int* nbr, const double* dl_blk,
                              int nblk, int n, double dt, const double* dt_dev, double rk_b,
                              double plm_theta, double gamma_law_index, int precision, void* stream);
-/* euler2d stage over the leaves of a 2:1-balanced quad-tree: guard fill
- * across level jumps (amr::extend_block over amr::get_or_create_block,
- * src/mesh_amr.hpp:300-393: same level copy, coarse neighbour injection,
- * fine neighbours 2x2 averaged, all on primitives) into pe_work
- * [nleaf][5][n+4][n+4], then the fused update with dl_leaf [nleaf]
- * (cell_spacing per level, src/mesh_geometry.cpp:168-171).  recipes
+/* euler2d stage over the leaves of a 2:1-balanced quad-tree, with the guard
+ * fill across level jumps of amr::extend_block over amr::get_or_create_block
+ * (src/mesh_amr.hpp:300-393: same level copy, coarse neighbour injection,
+ * fine neighbours 2x2 averaged, all on primitives) and cell spacing dl_leaf
+ * [nleaf] per leaf (cell_spacing, src/mesh_geometry.cpp:168-171).
+ * pe_work NULL: one fused launch reads guard zones through the recipes.
+ * pe_work [nleaf][5][n+4][n+4]: the extended primitive blocks are written
+ * there first (the reference's extend-then-update structure), then the
+ * update reads them.  Both give identical results.  n even, >= 16.  recipes
  * [nleaf][4][8] int (directions -x, +x, -y, +y):
  * {kind 0 same / 1 coarse / 2 fine, leaf, child00, child10, child01,
  *  child11, orthant_i, orthant_j}.  As the reference, no flux correction at

@@ -124,6 +124,12 @@ def lib():
             raise ImportError(
                 "mara3p_amd: HIP library %s is not built (run __graft_entry__.build() "
                 "or python mara3p_amd/build.py)" % LIBRARY_PATH)
+        # The library shares device pointers and streams with PyTorch, so it
+        # must bind to the same HIP runtime.  PyTorch ships its own
+        # libamdhip64.so.7; loading torch first makes that the process's copy
+        # (same soname), instead of the system one this library was linked
+        # against being loaded first and then picked up by torch as well.
+        import torch  # noqa: F401
         handle = ctypes.CDLL(LIBRARY_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

@@ -648,16 +648,24 @@ int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double
 {
     DeviceState* ds;
     if (int st = device_state(&ds)) return st;
-    if (!u || !out || !pe_work || !recipes || !dl_leaf || nleaf <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_amr_euler2d_stage: bad arguments");
+    if (!u || !out || !recipes || !dl_leaf || nleaf <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_amr_euler2d_stage: bad arguments");
     if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
+    if (n < 16 || n % 2) return fail(M3P_ERR_BAD_SHAPE, "m3p_amr_euler2d_stage: leaf size must be an even number >= 16");
     EulerParams prm{ gamma_law_index, plm_theta, hllc };
     hipStream_t s = (hipStream_t)stream;
-    const long ext = (long)(n + 4) * (n + 4) * nleaf;
-    hipLaunchKernelGGL(amr_guard_fill_kernel, dim3((ext + 255) / 256), dim3(256), 0, s, u, pe_work, recipes, nleaf, n, prm, ds->err);
-    if (int st = launch_status("amr_guard_fill_kernel")) return st;
     StageArgs a{};
     a.u0 = u0; a.out = out; a.dt_dev = dt_dev; a.dt = dt; a.dl_blk = dl_leaf; a.rk_b = rk_b;
     a.n = n; a.nblk = nleaf; a.err = ds->err;
+    if (!pe_work) {
+        // fused: guard zones read through the recipes inside the stage kernel
+        if (precision == M3P_FP64)
+            return launch_stage2d<EulerScheme<double>>(M3P_KIND_EULER2D, prm, TreeLoader<EulerScheme<double>>{u, recipes, n}, a, s);
+        return launch_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, TreeLoader<EulerScheme<float>>{u, recipes, n}, a, s);
+    }
+    // two passes: extended primitive blocks into pe_work, then the update
+    const long ext = (long)(n + 4) * (n + 4) * nleaf;
+    hipLaunchKernelGGL(amr_guard_fill_kernel, dim3((ext + 255) / 256), dim3(256), 0, s, u, pe_work, recipes, nleaf, n, prm, ds->err);
+    if (int st = launch_status("amr_guard_fill_kernel")) return st;
     if (precision == M3P_FP64)
         return launch_stage2d<EulerScheme<double>>(M3P_KIND_EULER2D, prm, ExtendedLoader<EulerScheme<double>>{u, pe_work, n}, a, s);
     return launch_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, ExtendedLoader<EulerScheme<float>>{u, pe_work, n}, a, s);

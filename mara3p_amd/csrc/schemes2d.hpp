@@ -23,7 +23,7 @@ struct TileCtx {
 template<typename R>
 struct EulerScheme {
     static constexpr int NC = 5;
-    static constexpr int kWavesPerEU = is_fp64<R>::value ? 2 : 4;   // occupancy floor (VGPR budget)
+    static constexpr int kWavesPerEU = is_fp64<R>::value ? 2 : 3;   // occupancy floor (VGPR budget)
     using Real = R;
     using Params = EulerParams;
     template<int H, int TJ> struct TileData {};
@@ -80,7 +80,7 @@ struct MinidiskParams {
 template<typename R>
 struct MinidiskScheme {
     static constexpr int NC = 3;
-    static constexpr int kWavesPerEU = is_fp64<R>::value ? 3 : 5;
+    static constexpr int kWavesPerEU = is_fp64<R>::value ? 3 : 4;
     using Real = R;
     using Params = MinidiskParams;
     // Vertex coordinates of the tile's rows/columns in FP64 (reference formula),

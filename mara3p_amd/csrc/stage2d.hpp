@@ -27,22 +27,37 @@ struct MeshLoader {
     const double* u;
     const int* nbr;
     int n;
+    // The four neighbours of the block being updated, read once per
+    // workgroup by bind() (wave-uniform, kept in scalar registers).
+    int bxm = 0, bxp = 0, bym = 0, byp = 0;
 
-    // Pointer to zone (r, c) of block b; rows and columns one block outside
-    // [0, n) are taken from the neighbouring block.
+    __device__ __forceinline__ void bind(int b)
+    {
+        bxm = nbr[4 * b + 0];
+        bxp = nbr[4 * b + 1];
+        bym = nbr[4 * b + 2];
+        byp = nbr[4 * b + 3];
+    }
+    // Pointer to zone (r, c) of the bound block b; rows and columns one
+    // block outside [0, n) are taken from the neighbouring block (a corner
+    // takes two hops through the table).
     __device__ __forceinline__ const double* zone(int b, int r, int c) const
     {
-        if (r < 0)        { b = nbr[4 * b + 0]; r += n; }
-        else if (r >= n)  { b = nbr[4 * b + 1]; r -= n; }
-        if (c < 0)        { b = nbr[4 * b + 2]; c += n; }
-        else if (c >= n)  { b = nbr[4 * b + 3]; c -= n; }
+        if (r < 0 || r >= n) {
+            const int bb = r < 0 ? bxm : bxp;
+            r += r < 0 ? n : -n;
+            if (c < 0)       { b = nbr[4 * bb + 2]; c += n; }
+            else if (c >= n) { b = nbr[4 * bb + 3]; c -= n; }
+            else b = bb;
+        } else if (c < 0)  { b = bym; c += n; }
+        else if (c >= n)   { b = byp; c -= n; }
         return u + ((size_t)b * S::NC * n + r) * n + c;
     }
-    // Start of row r of block b (wave-uniform); columns must lie in [0, n).
+    // Start of row r of the bound block b (wave-uniform); columns must lie in [0, n).
     __device__ __forceinline__ const double* row(int b, int r) const
     {
-        if (r < 0)        { b = nbr[4 * b + 0]; r += n; }
-        else if (r >= n)  { b = nbr[4 * b + 1]; r -= n; }
+        if (r < 0)        { b = bxm; r += n; }
+        else if (r >= n)  { b = bxp; r -= n; }
         return u + ((size_t)b * S::NC * n + r) * n;
     }
     __device__ __forceinline__ void cons_at(const double* z, double* uu) const
@@ -75,6 +90,17 @@ struct MeshLoader {
         cons_at(row(b, r) + c, uu);
         return S::recover(uu, prm, p);
     }
+    // Split load for software pipelining: fetch the raw conserved state of a
+    // row zone one iteration ahead, recover it when it is needed.
+    static constexpr bool kPrefetch = true;
+    __device__ __forceinline__ void fetch_row(int b, int r, int c, double* raw) const { cons_row(b, r, c, raw); }
+    __device__ __forceinline__ bool finish_row(int, int, int, const typename S::Params& prm, const double* raw,
+                                               double* uu, typename S::Real* p) const
+    {
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q) uu[q] = raw[q];
+        return S::recover(uu, prm, p);
+    }
 };
 
 // One block given as the reference's per-block arguments: uc [NC][n][n]
@@ -84,6 +110,8 @@ struct ExtendedLoader {
     const double* uc;
     const double* pe;
     int n;
+    static constexpr bool kPrefetch = false;
+    __device__ __forceinline__ void bind(int) {}
 
     // Block b of a list of blocks: uc + b NC n^2, pe + b NC (n+4)^2.
     __device__ __forceinline__ void cons(int b, int r, int c, double* uu) const
@@ -118,6 +146,107 @@ struct ExtendedLoader {
     }
 };
 
+// Leaves of a 2:1-balanced quad-tree, [nleaf][NC][n][n].  Guard zones are
+// read through a per-leaf recipe [nleaf][4][8] for the directions -x, +x,
+// -y, +y: {kind, leaf, child00, child10, child01, child11, orthant_i,
+// orthant_j}.  amr::get_or_create_block (src/mesh_amr.hpp:300-328) fused into
+// the load:
+//   kind 0  same-level leaf: its zone
+//   kind 1  coarser leaf (refine_block = upsample + split_block): the parent
+//           zone ((o n + i) / 2, (o n + j) / 2), piecewise-constant injection
+//   kind 2  four finer leaves (coarsen_blocks = tile_blocks + downsample):
+//           0.25 (((a00 + a01) + a10) + a11) of the four primitives
+// The tile march reads guard rows at interior columns and guard columns at
+// interior rows only, never a corner.
+template<class S>
+struct TreeLoader {
+    using R = typename S::Real;
+    const double* u;
+    const int* recipes;
+    int n;
+    __device__ __forceinline__ void bind(int) {}
+
+    __device__ __forceinline__ bool prim_of(int leaf, int r, int c, const typename S::Params& prm, R* p) const
+    {
+        double uu[S::NC];
+        const double* z = u + ((size_t)leaf * S::NC * n + r) * n + c;
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q) uu[q] = z[(size_t)q * n * n];
+        return S::recover(uu, prm, p);
+    }
+    // Zone (i, j) of the virtual neighbour block in direction d of leaf b.
+    __device__ __forceinline__ bool guard(int b, int d, int i, int j, const typename S::Params& prm, R* p) const
+    {
+        const int* rec = recipes + ((size_t)b * 4 + d) * 8;
+        const int kind = rec[0];
+        if (kind == 0) return prim_of(rec[1], i, j, prm, p);
+        if (kind == 1) return prim_of(rec[1], (rec[6] * n + i) >> 1, (rec[7] * n + j) >> 1, prm, p);
+        const int I = 2 * i, J = 2 * j;           // I, I+1 (J, J+1) lie in one child: n is even
+        const int ci = I >= n, cj = J >= n;
+        const int child = rec[2 + ci + 2 * cj];
+        const int li = I - ci * n, lj = J - cj * n;
+        R a[S::NC], b1[S::NC], c1[S::NC], e[S::NC];
+        bool ok = prim_of(child, li, lj, prm, a);
+        ok &= prim_of(child, li, lj + 1, prm, b1);
+        ok &= prim_of(child, li + 1, lj, prm, c1);
+        ok &= prim_of(child, li + 1, lj + 1, prm, e);
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q) p[q] = R(0.25) * (((a[q] + b1[q]) + c1[q]) + e[q]);
+        return ok;
+    }
+    // Interior row r, any column (guard columns come from the -y / +y recipes).
+    __device__ __forceinline__ bool prim(int b, int r, int c, const typename S::Params& prm, R* p) const
+    {
+        if (c < 0)  return guard(b, 2, r, c + n, prm, p);
+        if (c >= n) return guard(b, 3, r, c - n, prm, p);
+        return prim_of(b, r, c, prm, p);
+    }
+    // Interior column c, any row (guard rows come from the -x / +x recipes).
+    __device__ __forceinline__ bool prim_row(int b, int r, int c, const typename S::Params& prm, R* p) const
+    {
+        if (r < 0)  return guard(b, 0, r + n, c, prm, p);
+        if (r >= n) return guard(b, 1, r - n, c, prm, p);
+        return prim_of(b, r, c, prm, p);
+    }
+    __device__ __forceinline__ void cons_row(int b, int r, int c, double* uu) const
+    {
+        const double* z = u + ((size_t)b * S::NC * n + r) * n + c;
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q) uu[q] = z[(size_t)q * n * n];
+    }
+    // The conserved state is only kept for interior rows (guard rows are never updated).
+    __device__ __forceinline__ bool load_row(int b, int r, int c, const typename S::Params& prm,
+                                             double* uu, R* p) const
+    {
+        if (r >= 0 && r < n) {
+            cons_row(b, r, c, uu);
+            return S::recover(uu, prm, p);
+        }
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q) uu[q] = 0.0;
+        return prim_row(b, r, c, prm, p);
+    }
+    // Software pipelining as MeshLoader for interior rows; a guard row is
+    // assembled when it is needed.
+    static constexpr bool kPrefetch = true;
+    __device__ __forceinline__ void fetch_row(int b, int r, int c, double* raw) const
+    {
+        if (r < n) cons_row(b, r, c, raw);
+    }
+    __device__ __forceinline__ bool finish_row(int b, int r, int c, const typename S::Params& prm, const double* raw,
+                                               double* uu, R* p) const
+    {
+        if (r < n) {
+#pragma unroll
+            for (int q = 0; q < S::NC; ++q) uu[q] = raw[q];
+            return S::recover(uu, prm, p);
+        }
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q) uu[q] = 0.0;
+        return guard(b, 1, r - n, c, prm, p);
+    }
+};
+
 struct StageArgs {
     const double* u0;      // RK base state (same layout as out) or nullptr
     double* out;           // [nblk][NC][n][n]
@@ -138,7 +267,7 @@ struct StageArgs {
 // at the update (fewer registers).
 template<class S, class Ld, int H, int TJ, bool KEEPU>
 __global__ void __launch_bounds__(TJ) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU, 8)))
-stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
+stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
 {
     constexpr int NC = S::NC;
     using R = typename S::Real;
@@ -161,6 +290,8 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
     const int b = tc.b;
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
     const double dl = a.dl_blk ? a.dl_blk[b] : a.dl;
+    Ld ld = ld_arg;
+    ld.bind(b);
 
     __shared__ typename S::template TileData<H, TJ> td;
     __shared__ R hraw[7][NC][H];     // pre-pass primitives at 7 edge columns
@@ -231,12 +362,30 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
     // ---- march ------------------------------------------------------------
     // Edge lanes read their out-of-tile neighbours from the pre-pass arrays
     // through a selected address (no divergent branch).
+    // Loaders that support it are software-pipelined: the raw state of row
+    // k+3 and the RK base of row k are fetched at the top of iteration k, so
+    // their latency overlaps this iteration's arithmetic.
+    double Unx[NC] = {};
+    if constexpr (Ld::kPrefetch) ld.fetch_row(b, tc.k0 + 2, c, Unx);
     for (int kk = 0; kk < H; ++kk) {
         const int k = tc.k0 + kk;
         R Pc[NC], gb[NC], pl[NC], pr[NC], Fxn[NC], gy[NC], Fy[NC];
-        double Uc[NC];
-        if constexpr (KEEPU) ok &= ld.load_row(b, k + 2, c, prm, Uc, Pc);
-        else                 ok &= ld.prim_row(b, k + 2, c, prm, Pc);
+        double Uc[NC], U0[NC];
+        if constexpr (Ld::kPrefetch) {
+            double raw[NC];
+#pragma unroll
+            for (int m = 0; m < NC; ++m) raw[m] = Unx[m];
+            if (kk + 1 < H) ld.fetch_row(b, k + 3, c, Unx);
+            if (a.u0) {
+                const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
+#pragma unroll
+                for (int m = 0; m < NC; ++m) U0[m] = z0[(size_t)m * n * n];
+            }
+            ok &= ld.finish_row(b, k + 2, c, prm, raw, Uc, Pc);
+        } else {
+            if constexpr (KEEPU) ok &= ld.load_row(b, k + 2, c, prm, Uc, Pc);
+            else                 ok &= ld.prim_row(b, k + 2, c, prm, Pc);
+        }
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
             gb[m] = plm_half(Pa[m], Pb[m], Pc[m], theta);
@@ -285,9 +434,13 @@ stage2d_kernel(typename S::Params prm, Ld ld, StageArgs a)
         S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, dl, res);
         double* o = a.out + ((size_t)b * NC * n + k) * n + c;
         if (a.u0) {
-            const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
+            if constexpr (!Ld::kPrefetch) {
+                const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
-            for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = z0[(size_t)m * n * n] * a.rk_b + res[m] * (1.0 - a.rk_b);
+                for (int m = 0; m < NC; ++m) U0[m] = z0[(size_t)m * n * n];
+            }
+#pragma unroll
+            for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = U0[m] * a.rk_b + res[m] * (1.0 - a.rk_b);
         } else {
 #pragma unroll
             for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = res[m];

@@ -159,12 +159,13 @@ class Euler1DSolver(_StateSolver):
 class AmrEuler2DSolver(_StateSolver):
     """euler2d on the leaves of a 2:1-balanced quad-tree (module_euler.cpp:263-323
     over amr::extend_block, src/mesh_amr.hpp:300-393).  All leaves advance with
-    one dt; each uses its own level's cell spacing.  One guard-fill launch and
-    one fused stage launch per RK stage."""
+    one dt; each uses its own level's cell spacing.  fused=True: one launch
+    per RK stage reads guard zones through the leaf recipes; fused=False: a
+    guard-fill launch writes extended primitive blocks, then the update."""
     nc = 5
 
     def __init__(self, tree: QuadTree, block_size, gamma=5.0 / 3.0, plm_theta=1.5, precision="fp64",
-                 hllc=False, domain_size=1.0, device="cuda"):
+                 hllc=False, domain_size=1.0, fused=True, device="cuda"):
         self.tree = tree
         self.n = int(block_size)
         self.nleaf = len(tree.leaves)
@@ -172,7 +173,8 @@ class AmrEuler2DSolver(_StateSolver):
         self.precision = PRECISIONS[precision]
         n = self.n
         self._alloc((self.nleaf, 5, n, n), device)
-        self.pe = torch.zeros((self.nleaf, 5, n + 4, n + 4), dtype=torch.float64, device=self.device)
+        self.pe = None if fused else torch.zeros((self.nleaf, 5, n + 4, n + 4), dtype=torch.float64,
+                                                 device=self.device)
         self.recipes = torch.as_tensor(tree.recipes(), device=self.device)
         self.dl_host = domain_size / float(n) / (2.0 ** tree.levels().astype(np.float64))
         self.dl = torch.as_tensor(self.dl_host, dtype=torch.float64, device=self.device)

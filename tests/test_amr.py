@@ -191,12 +191,13 @@ def _to_ref_layout(u_dev):
 
 @pytest.mark.gpu
 @gpu
-@pytest.mark.parametrize("n,hllc,depth", [(16, 0, 5), (16, 1, 5), (32, 0, 4)])
-def test_gpu_amr_stage_fp64_matches_oracle(n, hllc, depth):
+@pytest.mark.parametrize("n,hllc,depth,fused", [(16, 0, 5, True), (16, 1, 5, True), (32, 0, 4, True),
+                                                (16, 0, 5, False), (32, 1, 4, False)])
+def test_gpu_amr_stage_fp64_matches_oracle(n, hllc, depth, fused):
     from mara3p_amd.solvers import AmrEuler2DSolver
     t = offset_tree(depth)
     u = _kh_leaves(t, n)
-    s = AmrEuler2DSolver(t, n, gamma=G, hllc=bool(hllc))
+    s = AmrEuler2DSolver(t, n, gamma=G, hllc=bool(hllc), fused=fused)
     s.set_state(_to_dev_layout(u))
     dt = 2e-4
     s.step(dt)
@@ -216,7 +217,7 @@ def test_gpu_amr_guard_fill_matches_oracle():
     n = 16
     t = offset_tree()
     u = _kh_leaves(t, n)
-    s = AmrEuler2DSolver(t, n, gamma=G)
+    s = AmrEuler2DSolver(t, n, gamma=G, fused=False)
     s.set_state(_to_dev_layout(u))
     s.stage(s.u, None, s.u1, 1e-4)
     torch.cuda.synchronize()
@@ -231,17 +232,23 @@ def test_gpu_amr_guard_fill_matches_oracle():
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_gpu_sedov_amr_mirror_symmetry(precision):
     """The nested tree and the Sedov state are symmetric under x -> -x; so is
-    the solution after CFL-limited steps through every kind of level jump."""
+    the solution after CFL-limited steps through every kind of level jump.
+    In FP64 the fused and two-pass guard fills agree exactly."""
     from mara3p_amd.problems import sedov_amr
     from mara3p_amd.solvers import AmrEuler2DSolver
     t = amr.nested_tree(5, 1.5)
     n = 16
     s = AmrEuler2DSolver(t, n, precision=precision)
+    s2 = AmrEuler2DSolver(t, n, precision=precision, fused=False)
     s.set_state(sedov_amr(t, n))
+    s2.set_state(sedov_amr(t, n))
     for _ in range(10):
         s.step_cfl(0.4)
+        s2.step_cfl(0.4)
     s.check_errors()
     u = s.state()
+    if precision == "fp64":
+        assert np.array_equal(u, s2.state())
     tol = 1e-10 if precision == "fp64" else 1e-5
     for k, (l, i, j) in enumerate(t.leaves):
         m = t.index[(l, (1 << l) - 1 - i, j)]
