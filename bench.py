@@ -12,12 +12,14 @@ One "step" = one RK2 step (two fused stage launches) over the whole mesh,
 state resident in HBM.  A zone-update is one zone through one RK2 step
 (the reference's kzps convention, src/app_problem.cpp:84-88).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload minidisk|kh|sedov_amr|ensemble]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload minidisk|kh|sedov_amr|ensemble|mhd]
 
 The other workloads are timed the same way for the record: kh = euler2d
 Kelvin-Helmholtz 4096^2; sedov_amr = euler2d Sedov blast on a 2:1 quad-tree
 of 1408 leaves of 64^2 (levels 3-6); ensemble = 512 independent 1D Riemann
-problems of 512-8192 zones in one work-list launch per stage.
+problems of 512-8192 zones in one work-list launch per stage; mhd = ideal
+MHD with constrained transport, Orszag-Tang at 256^3 (the stage time step
+recomputed on the device every stage).
 """
 import argparse
 import ctypes
@@ -40,11 +42,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["minidisk", "kh", "sedov_amr", "ensemble"], default="minidisk")
+    ap.add_argument("--workload", choices=["minidisk", "kh", "sedov_amr", "ensemble", "mhd"], default="minidisk")
     ap.add_argument("--l1-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every tile shape first and keep the fastest (default: the library's tile table)")
+    ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (for HBM counter calibration under rocprofv3)")
     return ap.parse_args()
@@ -90,10 +94,24 @@ def make_solver(workload, precision):
     return s, u0, (lambda t: dt), 5
 
 
-def timed_run(s, dt_of, steps, warmup, autotune=True):
+TILE_KIND = {"kh": 0, "minidisk": 1, "sedov_amr": 2}
+
+
+def tile_in_use(workload, precision):
+    """The stage kernel's tile shape (columns, rows) from the library's table."""
+    from mara3p_amd._lib import lib, PRECISIONS
+    if workload not in TILE_KIND:
+        return None
+    tj, h = ctypes.c_int(0), ctypes.c_int(0)
+    lib().m3p_get_stage2d_tile(TILE_KIND[workload], PRECISIONS[precision], ctypes.byref(tj), ctypes.byref(h))
+    return [tj.value, h.value]
+
+
+def timed_run(s, dt_of, steps, warmup, autotune=False):
     """Time `steps` RK2 steps; per-launch durations from HIP events on the launch stream."""
     import torch
-    tile = s.autotune() if autotune and hasattr(s, "autotune") else None
+    if autotune and hasattr(s, "autotune"):
+        s.autotune()
     for _ in range(warmup):
         s.step(dt_of(getattr(s, "time", 1.0)))
     torch.cuda.synchronize()
@@ -118,7 +136,7 @@ def timed_run(s, dt_of, steps, warmup, autotune=True):
     s.stage = orig_stage
     s.check_errors()
     durs = [a.elapsed_time(b) for a, b in ev]      # ms
-    return wall, float(np.mean(durs[0::2])), float(np.mean(durs[1::2])), tile
+    return wall, float(np.mean(durs[0::2])), float(np.mean(durs[1::2]))
 
 
 # ---------------------------------------------------------------------------
@@ -194,6 +212,52 @@ def traffic_from_profiles(workload, precision):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def bench_mhd(args):
+    """3D MHD CT at 256^3: K RK2 steps, per-launch times from HIP events
+    around each advance_mhd_ct stage (several kernels per stage)."""
+    import torch
+    from mara3p_amd import problems
+    from mara3p_amd.solvers import MhdSolver
+    N = 256
+    uc, b1, b2, b3 = problems.orszag_tang_3d(N, seed=1)
+    out = {}
+    for precision in ("mixed", "fp64"):
+        s = MhdSolver(N, precision=precision)
+        s.set_state(uc, b1, b2, b3)
+        for _ in range(args.warmup):
+            s.step()
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for e0, e1 in ev:
+            e0.record(stream)
+            s.step()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        s.check_errors()
+        step_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        zones = N ** 3
+        # RK2: stage 1 reads 8 fields and writes 8, stage 2 also reads the base: 5 x 8 x 8 B per zone-update
+        alg = 5 * 8 * 8 * zones
+        out[precision] = {"value": zones * args.steps / wall, "ms_per_step": 1e3 * wall / args.steps,
+                          "event_ms_per_step": step_ms, "hbm_frac": alg / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "max_div_b": s.max_div_b()}
+        del s
+        torch.cuda.synchronize()
+    m = out["mixed"]
+    result = {"metric": "zone-updates/s (3D MHD CT)", "value": m["value"], "unit": "zone-updates/s", "n_gpus": 1,
+              "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None, "dtype": "f32-flux/f64-state", "data": "synthetic",
+              "config": {"workload": "ideal MHD CT, Orszag-Tang 256^3 periodic, PLM+HLLE, RK2", "zones": N ** 3},
+              "roofline": {"bound": "hbm", "achieved": m["hbm_frac"] * HBM_PEAK_GBS, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": m["hbm_frac"], "traffic": None,
+                           "note": "per RK2 step over all kernels of both stages"},
+              "fp64": out["fp64"], "max_div_b_mixed": m["max_div_b"]}
+    print(json.dumps(result), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -201,6 +265,8 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.workload == "mhd":
+        return bench_mhd(args)
 
     if args.copy_calibration:
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
@@ -210,7 +276,8 @@ def main():
         del a, b
     s, u0, dt_of, nc = make_solver(args.workload, "mixed")
     zones = s.zones
-    wall, d1, d2, tile = timed_run(s, dt_of, args.steps, args.warmup, not args.no_autotune)
+    wall, d1, d2 = timed_run(s, dt_of, args.steps, args.warmup, args.autotune)
+    tile = tile_in_use(args.workload, "mixed")
     ms_per_step = 1e3 * wall / args.steps
     value = zones * args.steps / wall
     # algorithmic bytes: stage 1 reads U0 writes U1 (2 nc 8 B/zone), stage 2 reads U1, U0 writes U (3 nc 8 B/zone)
@@ -255,7 +322,8 @@ def main():
     if not args.no_fp64:
         s64, _, _, _ = make_solver(args.workload, "fp64")
         k64 = max(10, args.steps // 4)
-        wall64, e1, e2, tile64 = timed_run(s64, dt_of, k64, max(2, args.warmup // 4), not args.no_autotune)
+        wall64, e1, e2 = timed_run(s64, dt_of, k64, max(2, args.warmup // 4), args.autotune)
+        tile64 = tile_in_use(args.workload, "fp64")
         v64 = zones * k64 / wall64
         result["fp64"] = {"value": v64, "ms_per_step": 1e3 * wall64 / k64, "stage_ms": {"stage1": e1, "stage2": e2},
                           "tile": tile64,

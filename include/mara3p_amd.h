@@ -33,6 +33,7 @@ extern "C" {
 
 #define M3P_KIND_EULER2D   0   /* stage2d scheme kinds, for tile selection */
 #define M3P_KIND_MINIDISK  1
+#define M3P_KIND_AMR_EULER2D 2
 
 #define M3P_FP64   0   /* drop-in: reproduces the reference FP64 arithmetic */
 #define M3P_MIXED  1   /* FP32 reconstruction and fluxes, FP64 conserved state */

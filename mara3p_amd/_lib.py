@@ -22,6 +22,7 @@ M3P_FP64 = 0
 M3P_MIXED = 1
 M3P_KIND_EULER2D = 0
 M3P_KIND_MINIDISK = 1
+M3P_KIND_AMR_EULER2D = 2
 
 PRECISIONS = {"fp64": M3P_FP64, "mixed": M3P_MIXED}
 

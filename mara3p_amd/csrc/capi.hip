@@ -73,7 +73,10 @@ int launch_status(const char* what)
 struct TileShape { int tj, h; };
 constexpr TileShape kTileVariants[] = { {256, 32}, {256, 16}, {128, 32}, {64, 64}, {64, 32}, {64, 16}, {32, 32}, {16, 16} };
 constexpr int kNumVariants = sizeof(kTileVariants) / sizeof(kTileVariants[0]);
-TileShape g_tile[2][2] = { { {256, 16}, {256, 32} }, { {256, 16}, {256, 16} } };   // [kind][precision]
+constexpr int kNumKinds = 3;
+// [kind][precision]: defaults from sweeps of the benchmark meshes; the tree
+// kind runs 64^2 leaves, where 64 x 16 tiles give the most workgroups.
+TileShape g_tile[kNumKinds][2] = { { {256, 16}, {256, 16} }, { {256, 16}, {256, 16} }, { {64, 16}, {64, 16} } };
 
 bool tile_fits(TileShape v, int n) { return n % v.tj == 0 && n % v.h == 0 && v.h <= n; }
 
@@ -81,8 +84,11 @@ TileShape pick_tile(int kind, int precision, int n)
 {
     TileShape want = g_tile[kind][precision];
     if (tile_fits(want, n)) return want;
-    for (auto v : kTileVariants) if (tile_fits(v, n)) return v;
-    return TileShape{0, 0};
+    // widest fitting tile, then the fewest rows (most workgroups)
+    TileShape best{0, 0};
+    for (auto v : kTileVariants)
+        if (tile_fits(v, n) && (v.tj > best.tj || (v.tj == best.tj && v.h < best.h))) best = v;
+    return best;
 }
 
 template<class S, class Ld>
@@ -153,9 +159,13 @@ template<typename R>
 int launch_stage1d(const EulerParams& prm, const Stage1DArgs& a, hipStream_t s, int cus)
 {
     const int n = a.n;
+    // persistent grid: enough waves per CU for the kernel's register budget
+    // (FP64 ~120 VGPRs: 4 waves per SIMD; mixed ~64: 8)
+    const int waves_per_cu = is_fp64<R>::value ? 16 : 32;
     auto go = [&] (auto kern, int TJ) {
         const long items = (long)a.nblk * (n / TJ);
-        const int grid = (int)std::min<long>(items, (long)std::max(cus, 1) * 16);
+        const int wgs_per_cu = std::max(1, waves_per_cu / std::max(1, TJ / 64));
+        const int grid = (int)std::min<long>(items, (long)std::max(cus, 1) * wgs_per_cu);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(TJ), 0, s, prm, a);
         return launch_status("stage1d_kernel");
     };
@@ -659,16 +669,16 @@ int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double
     if (!pe_work) {
         // fused: guard zones read through the recipes inside the stage kernel
         if (precision == M3P_FP64)
-            return launch_stage2d<EulerScheme<double>>(M3P_KIND_EULER2D, prm, TreeLoader<EulerScheme<double>>{u, recipes, n}, a, s);
-        return launch_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, TreeLoader<EulerScheme<float>>{u, recipes, n}, a, s);
+            return launch_stage2d<EulerScheme<double>>(M3P_KIND_AMR_EULER2D, prm, TreeLoader<EulerScheme<double>>{u, recipes, n}, a, s);
+        return launch_stage2d<EulerScheme<float>>(M3P_KIND_AMR_EULER2D, prm, TreeLoader<EulerScheme<float>>{u, recipes, n}, a, s);
     }
     // two passes: extended primitive blocks into pe_work, then the update
     const long ext = (long)(n + 4) * (n + 4) * nleaf;
     hipLaunchKernelGGL(amr_guard_fill_kernel, dim3((ext + 255) / 256), dim3(256), 0, s, u, pe_work, recipes, nleaf, n, prm, ds->err);
     if (int st = launch_status("amr_guard_fill_kernel")) return st;
     if (precision == M3P_FP64)
-        return launch_stage2d<EulerScheme<double>>(M3P_KIND_EULER2D, prm, ExtendedLoader<EulerScheme<double>>{u, pe_work, n}, a, s);
-    return launch_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, ExtendedLoader<EulerScheme<float>>{u, pe_work, n}, a, s);
+        return launch_stage2d<EulerScheme<double>>(M3P_KIND_AMR_EULER2D, prm, ExtendedLoader<EulerScheme<double>>{u, pe_work, n}, a, s);
+    return launch_stage2d<EulerScheme<float>>(M3P_KIND_AMR_EULER2D, prm, ExtendedLoader<EulerScheme<float>>{u, pe_work, n}, a, s);
 }
 
 int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, const double* dl_blk,
@@ -723,7 +733,7 @@ int m3p_graph_destroy(void* graph_exec)
 
 int m3p_set_stage2d_tile(int kind, int precision, int tj, int h)
 {
-    if (kind < 0 || kind > 1 || precision < 0 || precision > 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad kind/precision");
+    if (kind < 0 || kind >= kNumKinds || precision < 0 || precision > 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad kind/precision");
     for (auto v : kTileVariants)
         if (v.tj == tj && v.h == h) { g_tile[kind][precision] = v; return M3P_OK; }
     return fail(M3P_ERR_BAD_ARGUMENT, "m3p: tile " + std::to_string(tj) + "x" + std::to_string(h) + " is not a compiled variant");
@@ -731,7 +741,7 @@ int m3p_set_stage2d_tile(int kind, int precision, int tj, int h)
 
 int m3p_get_stage2d_tile(int kind, int precision, int* tj, int* h)
 {
-    if (kind < 0 || kind > 1 || precision < 0 || precision > 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad kind/precision");
+    if (kind < 0 || kind >= kNumKinds || precision < 0 || precision > 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad kind/precision");
     if (tj) *tj = g_tile[kind][precision].tj;
     if (h) *h = g_tile[kind][precision].h;
     return M3P_OK;

@@ -42,7 +42,7 @@ __device__ __forceinline__ R plm(R yl, R y0, R yr, R theta)
 
 // Half the PLM slope, 0.5 * plm(...), used for the face states p -/+ 0.5 g.
 // FP64: the reference form times 0.5 (exact).  FP32: the same value from
-// three-operand min/max: a common sign gives sign * min|.|, else 0.
+// three-operand min/max/median: a common sign gives sign * min|.|, else 0.
 template<typename R>
 __device__ __forceinline__ R plm_half(R yl, R y0, R yr, R theta)
 {
@@ -53,7 +53,9 @@ __device__ __forceinline__ R plm_half(R yl, R y0, R yr, R theta)
         R a = (y0 - yl) * ht;
         R b = (yr - yl) * R(0.25);
         R c = (yr - y0) * ht;
-        return fmaxf(R(0), fminf(fminf(a, b), c)) + fminf(R(0), fmaxf(fmaxf(a, b), c));
+        // median of (min, max, 0): min if all are positive, max if all are
+        // negative, 0 when the signs differ
+        return __builtin_amdgcn_fmed3f(fminf(fminf(a, b), c), fmaxf(fmaxf(a, b), c), R(0));
     }
 }
 

@@ -93,6 +93,7 @@ struct MeshLoader {
     // Split load for software pipelining: fetch the raw conserved state of a
     // row zone one iteration ahead, recover it when it is needed.
     static constexpr bool kPrefetch = true;
+    static constexpr int kUnroll = 2;    // march unroll (fewer register rotations)
     __device__ __forceinline__ void fetch_row(int b, int r, int c, double* raw) const { cons_row(b, r, c, raw); }
     __device__ __forceinline__ bool finish_row(int, int, int, const typename S::Params& prm, const double* raw,
                                                double* uu, typename S::Real* p) const
@@ -111,6 +112,7 @@ struct ExtendedLoader {
     const double* pe;
     int n;
     static constexpr bool kPrefetch = false;
+    static constexpr int kUnroll = 1;
     __device__ __forceinline__ void bind(int) {}
 
     // Block b of a list of blocks: uc + b NC n^2, pe + b NC (n+4)^2.
@@ -229,6 +231,7 @@ struct TreeLoader {
     // Software pipelining as MeshLoader for interior rows; a guard row is
     // assembled when it is needed.
     static constexpr bool kPrefetch = true;
+    static constexpr int kUnroll = 1;
     __device__ __forceinline__ void fetch_row(int b, int r, int c, double* raw) const
     {
         if (r < n) cons_row(b, r, c, raw);
@@ -367,6 +370,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
     // their latency overlaps this iteration's arithmetic.
     double Unx[NC] = {};
     if constexpr (Ld::kPrefetch) ld.fetch_row(b, tc.k0 + 2, c, Unx);
+#pragma unroll (Ld::kUnroll)
     for (int kk = 0; kk < H; ++kk) {
         const int k = tc.k0 + kk;
         R Pc[NC], gb[NC], pl[NC], pr[NC], Fxn[NC], gy[NC], Fy[NC];

@@ -24,6 +24,33 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+# The compiled stage-kernel tile shapes (columns x rows), as in capi.hip.
+TILE_VARIANTS = ((256, 32), (256, 16), (128, 32), (64, 64), (64, 32), (64, 16), (32, 32), (16, 16))
+
+
+def autotune_tiles(kind, precision, n, run, reps=5):
+    """Time run() (one stage launch on the caller's arrays) under every tile
+    shape that fits block size n, keep the fastest for (kind, precision);
+    returns (tj, h, ms)."""
+    best = None
+    for tj, h in TILE_VARIANTS:
+        if n % tj or n % h:
+            continue
+        check(lib().m3p_set_stage2d_tile(kind, precision, tj, h))
+        run()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        if best is None or ms < best[2]:
+            best = (tj, h, ms)
+    check(lib().m3p_set_stage2d_tile(kind, precision, best[0], best[1]))
+    return best
+
+
 class _StateSolver:
     nc = None
 
@@ -200,6 +227,12 @@ class AmrEuler2DSolver(_StateSolver):
         self.stage(self.u, None, self.u1, 0.0, dt_dev=dt_dev)
         self.stage(self.u1, self.u, self.u2, 0.0, dt_dev=dt_dev)
         self.u, self.u2 = self.u2, self.u
+
+    def autotune(self, reps=5):
+        """Pick the fastest tile shape for this tree (blocking); returns (tj, h, ms)."""
+        dt = float(self.cfl_dt(0.1).item())
+        return autotune_tiles(_lib.M3P_KIND_AMR_EULER2D, self.precision, self.n,
+                              lambda: self.stage(self.u, None, self.u1, dt), reps)
 
     def conserved_totals_weighted(self):
         """Totals of U * dA over the leaves (host, fsum)."""
