@@ -71,7 +71,7 @@ int launch_status(const char* what)
 // are listed below.  The variant used for each (scheme, precision) is a
 // per-process setting chosen by m3p_set_stage2d_tile or m3p_autotune_*.
 struct TileShape { int tj, h; };
-constexpr TileShape kTileVariants[] = { {256, 32}, {256, 16}, {128, 32}, {64, 64}, {64, 32}, {64, 16}, {32, 32}, {16, 16} };
+constexpr TileShape kTileVariants[] = { {256, 128}, {256, 64}, {256, 32}, {256, 16}, {128, 64}, {128, 32}, {64, 64}, {64, 32}, {64, 16}, {32, 32}, {16, 16} };
 constexpr int kNumVariants = sizeof(kTileVariants) / sizeof(kTileVariants[0]);
 constexpr int kNumKinds = 3;
 // [kind][precision]: defaults from sweeps of the benchmark meshes; the tree
@@ -103,8 +103,11 @@ int launch_stage2d_shape(TileShape ts, const typename S::Params& prm, const Ld& 
         hipLaunchKernelGGL(kern, dim3(grid), dim3(ts.tj), 0, s, prm, ld, a);
         return launch_status("stage2d_kernel");
     };
+    if (ts.tj == 256 && ts.h == 128) return go(stage2d_kernel<S, Ld, 128, 256, K>);
+    if (ts.tj == 256 && ts.h == 64) return go(stage2d_kernel<S, Ld, 64, 256, K>);
     if (ts.tj == 256 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 256, K>);
     if (ts.tj == 256 && ts.h == 16) return go(stage2d_kernel<S, Ld, 16, 256, K>);
+    if (ts.tj == 128 && ts.h == 64) return go(stage2d_kernel<S, Ld, 64, 128, K>);
     if (ts.tj == 128 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 128, K>);
     if (ts.tj == 64 && ts.h == 64)  return go(stage2d_kernel<S, Ld, 64, 64, K>);
     if (ts.tj == 64 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 64, K>);

@@ -25,7 +25,8 @@ def _stream():
 
 
 # The compiled stage-kernel tile shapes (columns x rows), as in capi.hip.
-TILE_VARIANTS = ((256, 32), (256, 16), (128, 32), (64, 64), (64, 32), (64, 16), (32, 32), (16, 16))
+TILE_VARIANTS = ((256, 128), (256, 64), (256, 32), (256, 16), (128, 64), (128, 32), (64, 64), (64, 32), (64, 16),
+                 (32, 32), (16, 16))
 
 
 def autotune_tiles(kind, precision, n, run, reps=5):

@@ -150,6 +150,47 @@ def test_euler2d_mesh_rk2_fp64_exact(depth, n, hllc):
     assert np.array_equal(s.state(), to_device_layout(ref))
 
 
+def _with_tile(kind, precision, tj, h):
+    """Set a stage tile shape; returns a restore callback."""
+    from mara3p_amd._lib import lib
+    otj, oh = ctypes.c_int(0), ctypes.c_int(0)
+    lib().m3p_get_stage2d_tile(kind, precision, ctypes.byref(otj), ctypes.byref(oh))
+    assert lib().m3p_set_stage2d_tile(kind, precision, tj, h) == 0
+    return lambda: lib().m3p_set_stage2d_tile(kind, precision, otj.value, oh.value)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_every_tile_shape_gives_the_same_stage(precision):
+    """Every compiled tile shape computes the same face fluxes in the same
+    order, so a stage is bitwise independent of the tiling; FP64 also equals
+    the oracle."""
+    from mara3p_amd import problems
+    from mara3p_amd._lib import M3P_KIND_EULER2D, PRECISIONS
+    from mara3p_amd.mesh import to_device_layout, to_reference_layout
+    from mara3p_amd.solvers import Euler2DSolver, TILE_VARIANTS
+    depth, n = 1, 256
+    mesh, u0 = problems.kelvin_helmholtz(depth=depth, block_size=n, seed=11)
+    s = Euler2DSolver(depth, n, precision=precision)
+    dt = 0.4 * s.dl / 2.0
+    results = {}
+    for tj, h in TILE_VARIANTS:
+        restore = _with_tile(M3P_KIND_EULER2D, PRECISIONS[precision], tj, h)
+        try:
+            s.set_state(u0)
+            s.step(dt)
+            s.check_errors()
+            results[(tj, h)] = s.state()
+        finally:
+            restore()
+    first = next(iter(results.values()))
+    for k, v in results.items():
+        assert np.array_equal(v, first), k
+    if precision == "fp64":
+        ref = to_reference_layout(u0).copy()
+        assert orc().orc_euler2d_mesh_rk2(P(ref), mesh.nb, n, dt, s.dl, 1.5, 5.0 / 3.0, 0, 1, 8) == 0
+        assert np.array_equal(first, to_device_layout(ref))
+
+
 def test_minidisk_mesh_rk2_fp64():
     from mara3p_amd import MinidiskData, problems
     from mara3p_amd.mesh import to_device_layout, to_reference_layout
