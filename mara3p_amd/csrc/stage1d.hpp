@@ -45,11 +45,29 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
     __shared__ R G[NC][TJ + 2];   // [i] = half slope at zone c0 - 1 + i
     __shared__ R F[NC][TJ + 1];   // [i] = face c0 + i (left face of zone c0 + i)
 
+    // The lane's own zone of the next item is fetched while the current item
+    // is computed (software pipelining across the work list).
+    auto fetch = [&] (long it, double* raw) {
+        const double* src = a.u + (size_t)(it / tiles) * NC * n + (it % tiles) * TJ + t;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) raw[q] = src[(size_t)q * n];
+    };
+    double nxt[NC] = {};
+    if (blockIdx.x < items) fetch(blockIdx.x, nxt);
+
     bool ok = true;
     for (long item = blockIdx.x; item < items; item += gridDim.x) {
         const int b = (int)(item / tiles);
         const int c0 = (int)(item % tiles) * TJ;
         const double dl = a.dl_blk ? a.dl_blk[b] : a.dl;
+        double uz[NC], U0[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) uz[q] = nxt[q];
+        if (item + gridDim.x < items) fetch(item + gridDim.x, nxt);
+        if (a.u0) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) U0[q] = a.u0[(size_t)b * NC * n + (size_t)q * n + c0 + t];
+        }
         auto load = [&] (int i, R* p) {
             // zone index i in [-2, n+2) of block b
             if (a.pe) {
@@ -74,7 +92,8 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
         };
 
         R p[NC];
-        load(c0 + t, p);
+        if (a.pe) load(c0 + t, p);
+        else ok &= Euler<R>::recover(uz, prm, p);
         for (int q = 0; q < NC; ++q) P[q][t + 2] = p[q];
         if (t < 4) {
             int i = t < 2 ? c0 - 2 + t : c0 + TJ + (t - 2);
@@ -98,14 +117,13 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
         __syncthreads();
 
         const int z = c0 + t;
-        const double* uc = a.u + (size_t)b * NC * n + z;
         double* o = a.out + (size_t)b * NC * n + z;
         for (int q = 0; q < NC; ++q) {
             double dfx = double(F[q][t + 1]) - double(F[q][t]);
             double res;
-            if constexpr (is_fp64<R>::value) res = uc[(size_t)q * n] - dfx * dt / dl;
-            else res = uc[(size_t)q * n] - dfx * (dt / dl);
-            if (a.u0) res = a.u0[(size_t)b * NC * n + (size_t)q * n + z] * a.rk_b + res * (1.0 - a.rk_b);
+            if constexpr (is_fp64<R>::value) res = uz[q] - dfx * dt / dl;
+            else res = uz[q] - dfx * (dt / dl);
+            if (a.u0) res = U0[q] * a.rk_b + res * (1.0 - a.rk_b);
             o[(size_t)q * n] = res;
         }
         if (!ok) { report_error(a.err, 1, b, z); ok = true; }

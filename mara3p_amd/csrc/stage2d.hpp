@@ -234,19 +234,19 @@ struct TreeLoader {
     static constexpr int kUnroll = 1;
     __device__ __forceinline__ void fetch_row(int b, int r, int c, double* raw) const
     {
-        if (r < n) cons_row(b, r, c, raw);
+        if (r >= 0 && r < n) cons_row(b, r, c, raw);
     }
     __device__ __forceinline__ bool finish_row(int b, int r, int c, const typename S::Params& prm, const double* raw,
                                                double* uu, R* p) const
     {
-        if (r < n) {
+        if (r >= 0 && r < n) {
 #pragma unroll
             for (int q = 0; q < S::NC; ++q) uu[q] = raw[q];
             return S::recover(uu, prm, p);
         }
 #pragma unroll
         for (int q = 0; q < S::NC; ++q) uu[q] = 0.0;
-        return guard(b, 1, r - n, c, prm, p);
+        return r < 0 ? guard(b, 0, r + n, c, prm, p) : guard(b, 1, r - n, c, prm, p);
     }
 };
 
@@ -333,129 +333,158 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
         for (int m = 0; m < NC; ++m) hF[m][r] = f[m];
     }
 
-    // ---- x warm-up: rows k0-2 .. k0+1, flux at face k0 ------------------
-    R Pa[NC], Pb[NC], ga[NC];   // ga: half slope at the current row
-    double Fxp[NC];             // x flux at the current row's lower face
-    double Ua[NC], Ub[NC];
-    {
-        R Pm2[NC], Pm1[NC], gm[NC], pl[NC], pr[NC];
-        ok &= ld.prim_row(b, tc.k0 - 2, c, prm, Pm2);
-        ok &= ld.prim_row(b, tc.k0 - 1, c, prm, Pm1);
-        if constexpr (KEEPU) {
-            ok &= ld.load_row(b, tc.k0, c, prm, Ua, Pa);
-            ok &= ld.load_row(b, tc.k0 + 1, c, prm, Ub, Pb);
-        } else {
-            ok &= ld.prim_row(b, tc.k0, c, prm, Pa);
-            ok &= ld.prim_row(b, tc.k0 + 1, c, prm, Pb);
-        }
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            gm[m] = plm_half(Pm2[m], Pm1[m], Pa[m], theta);
-            ga[m] = plm_half(Pm1[m], Pa[m], Pb[m], theta);
-            pl[m] = Pm1[m] + gm[m];
-            pr[m] = Pa[m] - ga[m];
-        }
-        R f[NC];
-        S::template face<1>(pl, pr, prm, td, tc, tc.k0, c, f);
-#pragma unroll
-        for (int m = 0; m < NC; ++m) Fxp[m] = double(f[m]);
-    }
-    __syncthreads();
-
     // ---- march ------------------------------------------------------------
+    // Tiles in even tile rows march down (increasing row index), odd ones up,
+    // so two vertically adjacent tiles reach the rows they share at about the
+    // same time (both at their start or both at their end) and the second
+    // read of those halo rows hits L2.  Every slope, face state and face flux
+    // is computed from the same operands in the same order in both
+    // directions, so the results do not depend on the direction.
+    //
     // Edge lanes read their out-of-tile neighbours from the pre-pass arrays
-    // through a selected address (no divergent branch).
-    // Loaders that support it are software-pipelined: the raw state of row
-    // k+3 and the RK base of row k are fetched at the top of iteration k, so
-    // their latency overlaps this iteration's arithmetic.
-    double Unx[NC] = {};
-    if constexpr (Ld::kPrefetch) ld.fetch_row(b, tc.k0 + 2, c, Unx);
+    // through a selected address (no divergent branch).  Loaders that
+    // support it are software-pipelined: the raw state of the row three
+    // ahead and the RK base of the current row are fetched at the top of an
+    // iteration, so their latency overlaps the iteration's arithmetic.
+    auto march = [&] (auto dir) {
+        constexpr int D = decltype(dir)::value;
+        const int kfirst = D > 0 ? tc.k0 : tc.k0 + H - 1;
+
+        // warm-up: rows kfirst - 2D .. kfirst + D, flux at the face behind kfirst
+        R Pa[NC], Pb[NC], ga[NC];   // current row, next row, half slope at the current row
+        double Fb[NC];              // x flux at the face behind the current row
+        double Ua[NC], Ub[NC];
+        {
+            R Pm2[NC], Pm1[NC], gm[NC], pl[NC], pr[NC];
+            ok &= ld.prim_row(b, kfirst - 2 * D, c, prm, Pm2);
+            ok &= ld.prim_row(b, kfirst - D, c, prm, Pm1);
+            if constexpr (KEEPU) {
+                ok &= ld.load_row(b, kfirst, c, prm, Ua, Pa);
+                ok &= ld.load_row(b, kfirst + D, c, prm, Ub, Pb);
+            } else {
+                ok &= ld.prim_row(b, kfirst, c, prm, Pa);
+                ok &= ld.prim_row(b, kfirst + D, c, prm, Pb);
+            }
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+                if constexpr (D > 0) {
+                    gm[m] = plm_half(Pm2[m], Pm1[m], Pa[m], theta);
+                    ga[m] = plm_half(Pm1[m], Pa[m], Pb[m], theta);
+                    pl[m] = Pm1[m] + gm[m];
+                    pr[m] = Pa[m] - ga[m];
+                } else {
+                    gm[m] = plm_half(Pa[m], Pm1[m], Pm2[m], theta);
+                    ga[m] = plm_half(Pb[m], Pa[m], Pm1[m], theta);
+                    pl[m] = Pa[m] + ga[m];
+                    pr[m] = Pm1[m] - gm[m];
+                }
+            }
+            R f[NC];
+            S::template face<1>(pl, pr, prm, td, tc, D > 0 ? kfirst : kfirst + 1, c, f);
+#pragma unroll
+            for (int m = 0; m < NC; ++m) Fb[m] = double(f[m]);
+        }
+        __syncthreads();
+
+        double Unx[NC] = {};
+        if constexpr (Ld::kPrefetch) ld.fetch_row(b, kfirst + 2 * D, c, Unx);
 #pragma unroll (Ld::kUnroll)
-    for (int kk = 0; kk < H; ++kk) {
-        const int k = tc.k0 + kk;
-        R Pc[NC], gb[NC], pl[NC], pr[NC], Fxn[NC], gy[NC], Fy[NC];
-        double Uc[NC], U0[NC];
-        if constexpr (Ld::kPrefetch) {
-            double raw[NC];
+        for (int kk = 0; kk < H; ++kk) {
+            const int k = kfirst + D * kk;
+            const int kl = k - tc.k0;
+            R Pc[NC], gb[NC], pl[NC], pr[NC], Fa[NC], gy[NC], Fy[NC];
+            double Uc[NC], U0[NC];
+            if constexpr (Ld::kPrefetch) {
+                double raw[NC];
 #pragma unroll
-            for (int m = 0; m < NC; ++m) raw[m] = Unx[m];
-            if (kk + 1 < H) ld.fetch_row(b, k + 3, c, Unx);
+                for (int m = 0; m < NC; ++m) raw[m] = Unx[m];
+                if (kk + 1 < H) ld.fetch_row(b, k + 3 * D, c, Unx);
+                if (a.u0) {
+                    const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) U0[m] = z0[(size_t)m * n * n];
+                }
+                ok &= ld.finish_row(b, k + 2 * D, c, prm, raw, Uc, Pc);
+            } else {
+                if constexpr (KEEPU) ok &= ld.load_row(b, k + 2 * D, c, prm, Uc, Pc);
+                else                 ok &= ld.prim_row(b, k + 2 * D, c, prm, Pc);
+            }
+            // slope at row k + D, flux at the face between rows k and k + D
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+                if constexpr (D > 0) {
+                    gb[m] = plm_half(Pa[m], Pb[m], Pc[m], theta);
+                    pl[m] = Pa[m] + ga[m];
+                    pr[m] = Pb[m] - gb[m];
+                } else {
+                    gb[m] = plm_half(Pc[m], Pb[m], Pa[m], theta);
+                    pl[m] = Pb[m] + gb[m];
+                    pr[m] = Pa[m] - ga[m];
+                }
+            }
+            S::template face<1>(pl, pr, prm, td, tc, D > 0 ? k + 1 : k, c, Fa);
+
+#pragma unroll
+            for (int m = 0; m < NC; ++m) rowP[m][t + 1] = Pa[m];
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+                const R* left = t == 0 ? &hraw[1][m][kl] : &rowP[m][t];
+                const R* right = t == TJ - 1 ? &hraw[5][m][kl] : &rowP[m][t + 2];
+                gy[m] = plm_half(*left, Pa[m], *right, theta);
+                rowG[m][t + 1] = gy[m];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+                const R* lp = t == 0 ? &hraw[1][m][kl] : &rowP[m][t];
+                const R* lg = t == 0 ? &hG[m][kl] : &rowG[m][t];
+                pl[m] = *lp + *lg;
+                pr[m] = Pa[m] - gy[m];
+            }
+            S::template face<2>(pl, pr, prm, td, tc, k, c, Fy);
+#pragma unroll
+            for (int m = 0; m < NC; ++m) rowF[m][t] = Fy[m];
+            __syncthreads();
+
+            double dfx[NC], dfy[NC], uu[NC], res[NC], fa[NC];
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+                const R* rf = t == TJ - 1 ? &hF[m][kl] : &rowF[m][t + 1];
+                fa[m] = double(Fa[m]);
+                dfx[m] = D > 0 ? fa[m] - Fb[m] : Fb[m] - fa[m];
+                dfy[m] = double(*rf) - double(Fy[m]);
+            }
+            if constexpr (KEEPU) {
+#pragma unroll
+                for (int m = 0; m < NC; ++m) uu[m] = Ua[m];
+            } else {
+                ld.cons_row(b, k, c, uu);
+            }
+            S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, dl, res);
+            double* o = a.out + ((size_t)b * NC * n + k) * n + c;
             if (a.u0) {
-                const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
+                if constexpr (!Ld::kPrefetch) {
+                    const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
-                for (int m = 0; m < NC; ++m) U0[m] = z0[(size_t)m * n * n];
-            }
-            ok &= ld.finish_row(b, k + 2, c, prm, raw, Uc, Pc);
-        } else {
-            if constexpr (KEEPU) ok &= ld.load_row(b, k + 2, c, prm, Uc, Pc);
-            else                 ok &= ld.prim_row(b, k + 2, c, prm, Pc);
-        }
+                    for (int m = 0; m < NC; ++m) U0[m] = z0[(size_t)m * n * n];
+                }
 #pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            gb[m] = plm_half(Pa[m], Pb[m], Pc[m], theta);
-            pl[m] = Pa[m] + ga[m];
-            pr[m] = Pb[m] - gb[m];
-        }
-        S::template face<1>(pl, pr, prm, td, tc, k + 1, c, Fxn);
-
+                for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = U0[m] * a.rk_b + res[m] * (1.0 - a.rk_b);
+            } else {
 #pragma unroll
-        for (int m = 0; m < NC; ++m) rowP[m][t + 1] = Pa[m];
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            const R* left = t == 0 ? &hraw[1][m][kk] : &rowP[m][t];
-            const R* right = t == TJ - 1 ? &hraw[5][m][kk] : &rowP[m][t + 2];
-            gy[m] = plm_half(*left, Pa[m], *right, theta);
-            rowG[m][t + 1] = gy[m];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            const R* lp = t == 0 ? &hraw[1][m][kk] : &rowP[m][t];
-            const R* lg = t == 0 ? &hG[m][kk] : &rowG[m][t];
-            pl[m] = *lp + *lg;
-            pr[m] = Pa[m] - gy[m];
-        }
-        S::template face<2>(pl, pr, prm, td, tc, k, c, Fy);
-#pragma unroll
-        for (int m = 0; m < NC; ++m) rowF[m][t] = Fy[m];
-        __syncthreads();
-
-        double dfx[NC], dfy[NC], uu[NC], res[NC], fxn[NC];
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            const R* rf = t == TJ - 1 ? &hF[m][kk] : &rowF[m][t + 1];
-            fxn[m] = double(Fxn[m]);
-            dfx[m] = fxn[m] - Fxp[m];
-            dfy[m] = double(*rf) - double(Fy[m]);
-        }
-        if constexpr (KEEPU) {
-#pragma unroll
-            for (int m = 0; m < NC; ++m) uu[m] = Ua[m];
-        } else {
-            ld.cons_row(b, k, c, uu);
-        }
-        S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, dl, res);
-        double* o = a.out + ((size_t)b * NC * n + k) * n + c;
-        if (a.u0) {
-            if constexpr (!Ld::kPrefetch) {
-                const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
-#pragma unroll
-                for (int m = 0; m < NC; ++m) U0[m] = z0[(size_t)m * n * n];
+                for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = res[m];
             }
 #pragma unroll
-            for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = U0[m] * a.rk_b + res[m] * (1.0 - a.rk_b);
-        } else {
+            for (int m = 0; m < NC; ++m) { Pa[m] = Pb[m]; Pb[m] = Pc[m]; ga[m] = gb[m]; Fb[m] = fa[m]; }
+            if constexpr (KEEPU) {
 #pragma unroll
-            for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = res[m];
+                for (int m = 0; m < NC; ++m) { Ua[m] = Ub[m]; Ub[m] = Uc[m]; }
+            }
         }
-#pragma unroll
-        for (int m = 0; m < NC; ++m) { Pa[m] = Pb[m]; Pb[m] = Pc[m]; ga[m] = gb[m]; Fxp[m] = fxn[m]; }
-        if constexpr (KEEPU) {
-#pragma unroll
-            for (int m = 0; m < NC; ++m) { Ua[m] = Ub[m]; Ub[m] = Uc[m]; }
-        }
-    }
+    };
+    if (((rem / tiles_j) & 1) == 0) march(std::integral_constant<int, 1>{});
+    else                            march(std::integral_constant<int, -1>{});
     if (!ok) report_error(a.err, 1, b, 0);
 }
 
