@@ -67,7 +67,7 @@ typedef struct {
     int    depth;
     double omega_frame;
     double mach_number;
-    double kinematic_viscosity;   /* must be 0: viscous fluxes not built */
+    double kinematic_viscosity;   /* viscous stress when != 0 (per-zone kernel) */
     double domain_radius;
     double softening_length;
     double eccentricity;
@@ -86,13 +86,15 @@ int m3p_minidisk_updated_conserved(const double* uc, const double* pe, double* o
                                    int level, int bi, int bj, double time, double dt,
                                    const m3p_minidisk_data* data /* host */, int precision, void* stream);
 
-/* locally_isothermal::updated_conserved, src/module_locally_isothermal.cpp:182-212
- * with zero viscosity.  The reference's temperature mapping is a host
- * callback; callers evaluate it at mesh_geometry.face_coordinates and pass
- * cs2_x [(n+1)][n] and cs2_y [n][(n+1)] (device). */
+/* locally_isothermal::updated_conserved, src/module_locally_isothermal.cpp:182-212,
+ * with the viscous stress of godunov_and_viscous_fluxes (:106-174) when
+ * kinematic_viscosity != 0 (then pe's corner zones are read too).  The
+ * reference's temperature mapping is a host callback; callers evaluate it at
+ * mesh_geometry.face_coordinates and pass cs2_x [(n+1)][n] and cs2_y
+ * [n][(n+1)] (device). */
 int m3p_locally_isothermal_updated_conserved(const double* uc, const double* pe, double* out, int n,
                                    double dt, double dl, const double* cs2_x, const double* cs2_y,
-                                   int precision, void* stream);
+                                   double kinematic_viscosity, int precision, void* stream);
 
 /* Block helpers of the module API --------------------------------------- */
 /* recover_primitive_array, src/module_euler.cpp:84-89 (euler1d) and :240-245

@@ -7,6 +7,7 @@
 #include <string>
 #include "mara3p_amd.h"
 #include "amr.hpp"
+#include "viscous2d.hpp"
 #include "elementwise.hpp"
 #include "mhd.hpp"
 #include "srhd.hpp"
@@ -183,6 +184,14 @@ int launch_stage1d(const EulerParams& prm, const Stage1DArgs& a, hipStream_t s, 
     return fail(M3P_ERR_BAD_SHAPE, "m3p: 1D block size must be 16, 32, 64, 128 or a multiple of 256 (got " + std::to_string(n) + ")");
 }
 
+template<class S, class Ld>
+int launch_visc(const typename S::Params& prm, const Ld& ld, const StageArgs& a, double nu, hipStream_t s)
+{
+    const long zones = (long)a.nblk * a.n * a.n;
+    hipLaunchKernelGGL((iso_visc_stage_kernel<S, Ld>), dim3((zones + 255) / 256), dim3(256), 0, s, prm, ld, a, nu);
+    return launch_status("iso_visc_stage_kernel");
+}
+
 MinidiskParams minidisk_params(const m3p_minidisk_data* d, double time)
 {
     MinidiskParams p;
@@ -206,8 +215,7 @@ MinidiskParams minidisk_params(const m3p_minidisk_data* d, double time)
 int check_minidisk_data(const m3p_minidisk_data* d)
 {
     if (!d) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: minidisk data is NULL");
-    if (d->kinematic_viscosity != 0.0)
-        return fail(M3P_ERR_BAD_ARGUMENT, "m3p: minidisk viscous fluxes are not implemented (nu must be 0)");
+    if (!(d->kinematic_viscosity >= 0.0)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: negative kinematic viscosity");
     if (d->depth < 0 || d->depth > 20) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad minidisk depth");
     return M3P_OK;
 }
@@ -416,8 +424,13 @@ int m3p_minidisk_updated_conserved(const double* uc, const double* pe, double* o
     // cell_size, problems/minidisk_scheme.cpp:78-81
     a.dl = 2.0 * data->domain_radius / double(data->block_size) / double(1 << level);
     hipStream_t s = (hipStream_t)stream;
+    const double nu = data->kinematic_viscosity;
     int st;
-    if (precision == M3P_FP64)
+    if (nu != 0.0)
+        st = precision == M3P_FP64
+           ? launch_visc<MinidiskScheme<double>>(prm, ExtendedLoader<MinidiskScheme<double>>{uc, pe, n}, a, nu, s)
+           : launch_visc<MinidiskScheme<float>>(prm, ExtendedLoader<MinidiskScheme<float>>{uc, pe, n}, a, nu, s);
+    else if (precision == M3P_FP64)
         st = launch_stage2d<MinidiskScheme<double>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<MinidiskScheme<double>>{uc, pe, n}, a, s);
     else
         st = launch_stage2d<MinidiskScheme<float>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<MinidiskScheme<float>>{uc, pe, n}, a, s);
@@ -459,6 +472,11 @@ int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int
     a.n = n; a.nblk = nblk; a.err = ds->err;
     a.dl = 2.0 * data->domain_radius / double(data->block_size) / double(1 << data->depth);
     hipStream_t s = (hipStream_t)stream;
+    const double nu = data->kinematic_viscosity;
+    if (nu != 0.0)
+        return precision == M3P_FP64
+             ? launch_visc<MinidiskScheme<double>>(prm, MeshLoader<MinidiskScheme<double>>{u, nbr, n}, a, nu, s)
+             : launch_visc<MinidiskScheme<float>>(prm, MeshLoader<MinidiskScheme<float>>{u, nbr, n}, a, nu, s);
     if (precision == M3P_FP64)
         return launch_stage2d<MinidiskScheme<double>>(M3P_KIND_MINIDISK, prm, MeshLoader<MinidiskScheme<double>>{u, nbr, n}, a, s);
     return launch_stage2d<MinidiskScheme<float>>(M3P_KIND_MINIDISK, prm, MeshLoader<MinidiskScheme<float>>{u, nbr, n}, a, s);
@@ -785,17 +803,22 @@ int m3p_autotune_minidisk(const double* u, double* out, const int* nbr, const in
 
 int m3p_locally_isothermal_updated_conserved(const double* uc, const double* pe, double* out, int n,
                                    double dt, double dl, const double* cs2_x, const double* cs2_y,
-                                   int precision, void* stream)
+                                   double kinematic_viscosity, int precision, void* stream)
 {
     DeviceState* ds;
     if (int st = device_state(&ds)) return st;
     if (!uc || !pe || !out || !cs2_x || !cs2_y) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: NULL array");
+    if (!(kinematic_viscosity >= 0.0)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: negative kinematic viscosity");
     IsoTableParams prm{ cs2_x, cs2_y, n };
     StageArgs a{};
     a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = ds->err;
     hipStream_t s = (hipStream_t)stream;
     int st;
-    if (precision == M3P_FP64)
+    if (kinematic_viscosity != 0.0)
+        st = precision == M3P_FP64
+           ? launch_visc<IsoTableScheme<double>>(prm, ExtendedLoader<IsoTableScheme<double>>{uc, pe, n}, a, kinematic_viscosity, s)
+           : launch_visc<IsoTableScheme<float>>(prm, ExtendedLoader<IsoTableScheme<float>>{uc, pe, n}, a, kinematic_viscosity, s);
+    else if (precision == M3P_FP64)
         st = launch_stage2d<IsoTableScheme<double>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<IsoTableScheme<double>>{uc, pe, n}, a, s);
     else
         st = launch_stage2d<IsoTableScheme<float>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<IsoTableScheme<float>>{uc, pe, n}, a, s);

@@ -124,6 +124,22 @@ struct MinidiskScheme {
         }
     }
 
+    // One zone's TileData (k0 = row, c0 = column): the per-zone kernels.
+    __device__ static __forceinline__ void setup_zone(TileData<1, 1>& td, const Params& prm, const TileCtx& tc)
+    {
+#pragma unroll
+        for (int t = 0; t <= 1; ++t) {
+            double v = vertex(prm.R, tc.n, prm.level, tc.bi, tc.k0 + t);
+            td.xv[t] = v;
+            td.xvf[t] = float(v);
+            double w = vertex(prm.R, tc.n, prm.level, tc.bj, tc.c0 + t);
+            td.yv[t] = w;
+            td.yvf[t] = float(w);
+        }
+        td.xcf[0] = float(0.5 * (td.xv[0] + td.xv[1]));
+        td.ycf[0] = float(0.5 * (td.yv[0] + td.yv[1]));
+    }
+
     // sound_speed_squared :127-132 at face_coordinates :134-151
     template<int AXIS, int H, int TJ>
     __device__ static __forceinline__ void face(const R* pl, const R* pr, const Params& prm,
@@ -276,6 +292,7 @@ struct IsoTableScheme {
     }
     template<int H, int TJ>
     __device__ static __forceinline__ void setup(TileData<H, TJ>&, const Params&, const TileCtx&) {}
+    __device__ static __forceinline__ void setup_zone(TileData<1, 1>&, const Params&, const TileCtx&) {}
 
     template<int AXIS, int H, int TJ>
     __device__ static __forceinline__ void face(const R* pl, const R* pr, const Params& prm,

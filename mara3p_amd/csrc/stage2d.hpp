@@ -39,16 +39,21 @@ struct MeshLoader {
         byp = nbr[4 * b + 3];
     }
     // Pointer to zone (r, c) of the bound block b; rows and columns one
-    // block outside [0, n) are taken from the neighbouring block (a corner
-    // takes two hops through the table).
+    // block outside [0, n) are taken from the neighbouring block.  Corners
+    // take two hops through the table and follow the reference's
+    // extend_block (src/mesh_amr.hpp:360-393, problems/minidisk.cpp:87-120):
+    // its corner blocks c02 = prev_on(next_on(b, 0), 1) and c20 =
+    // next_on(prev_on(b, 0), 1) sit at the (x-prev, y-next) and (x-next,
+    // y-prev) corners, i.e. those two corners read the diagonally opposite
+    // neighbour.  Only the viscous kernels read corners.
     __device__ __forceinline__ const double* zone(int b, int r, int c) const
     {
         if (r < 0 || r >= n) {
-            const int bb = r < 0 ? bxm : bxp;
-            r += r < 0 ? n : -n;
-            if (c < 0)       { b = nbr[4 * bb + 2]; c += n; }
-            else if (c >= n) { b = nbr[4 * bb + 3]; c -= n; }
-            else b = bb;
+            const bool lo = r < 0;
+            r += lo ? n : -n;
+            if (c < 0)       { b = lo ? nbr[4 * bxm + 2] : nbr[4 * bxm + 3]; c += n; }
+            else if (c >= n) { b = lo ? nbr[4 * bxp + 2] : nbr[4 * bxp + 3]; c -= n; }
+            else b = lo ? bxm : bxp;
         } else if (c < 0)  { b = bym; c += n; }
         else if (c >= n)   { b = byp; c -= n; }
         return u + ((size_t)b * S::NC * n + r) * n + c;

@@ -342,8 +342,8 @@ inline conserved_array_t updated_conserved(conserved_array_t uc, primitive_array
     const std::size_t n = uc.shape(0);
     if (pe.shape(0) != n + 4 || pe.shape(1) != n + 4)
         throw std::invalid_argument("locally_isothermal::updated_conserved (pe must be uc extended by 2 guard zones)");
-    if (kinematic_viscosity != 0.0)
-        throw std::invalid_argument("locally_isothermal::updated_conserved (viscous fluxes are not built)");
+    if (kinematic_viscosity < 0.0)
+        throw std::invalid_argument("locally_isothermal::updated_conserved (negative kinematic viscosity)");
     std::vector<double> cx((n + 1) * n), cy(n * (n + 1));
     for (std::size_t i = 0; i <= n; ++i)
         for (std::size_t j = 0; j < n; ++j) cx[i * n + j] = temperature(mesh_geometry.face_coordinate(block, 0, i, j));
@@ -355,7 +355,7 @@ inline conserved_array_t updated_conserved(conserved_array_t uc, primitive_array
     hip_check(hipMemcpy(dcx.ptr, cx.data(), cx.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
     hip_check(hipMemcpy(dcy.ptr, cy.data(), cy.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
     check(m3p_locally_isothermal_updated_conserved(du.ptr, dp.ptr, dout.ptr, int(n), dt, mesh_geometry.cell_spacing(block),
-                                                   dcx.ptr, dcy.ptr, M3P_FP64, nullptr));
+                                                   dcx.ptr, dcy.ptr, kinematic_viscosity, M3P_FP64, nullptr));
     conserved_array_t out(uc.shape());
     download_soa(dout.ptr, n * n, out.data());
     return out;

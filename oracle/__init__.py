@@ -47,7 +47,7 @@ _SIG = {
     "orc_two_body_acceleration": (None, [_D, _D, _D, _D, _D, _D, _P]),
     "orc_euler1d_updated_conserved": (_I, [_P, _P, _I, _D, _D, _D, _D, _I, _I, _P]),
     "orc_euler2d_updated_conserved": (_I, [_P, _P, _I, _I, _D, _D, _D, _D, _I, _P]),
-    "orc_locally_isothermal_updated_conserved": (_I, [_P, _P, _I, _I, _D, _D, _P, _P, _P]),
+    "orc_locally_isothermal_updated_conserved": (_I, [_P, _P, _I, _I, _D, _D, _P, _P, _D, _P]),
     "orc_minidisk_updated_conserved": (_I, [_P, _P, ctypes.POINTER(OrcMinidisk), _I, _L, _L, _D, _D, _P]),
     "orc_minidisk_initial_conserved": (None, [ctypes.POINTER(OrcMinidisk), _I, _L, _L, _P]),
     "orc_minidisk_cell_coordinates": (None, [ctypes.POINTER(OrcMinidisk), _I, _L, _L, _P, _P]),

@@ -283,8 +283,30 @@ static double unit_potential(double x, double y, double rs)
 
 typedef void (*face_flux_fn)(void* ctx, const double* pl, const double* pr, int axis, int i, int j, double* f);
 
+/* Viscous stress of the isothermal schemes (nc = 3: Sigma, vx, vy), added to
+ * the Godunov flux when visc = {nu, dl} is given:
+ * godunov_and_viscous_fluxes, src/module_locally_isothermal.cpp:106-174 and
+ * problems/minidisk_scheme.cpp:220-287.  With the reference's face states
+ * pl = (p - g/2)[1:] (upper cell) and pr = (p + g/2)[:-1] (lower cell):
+ *   mu = nu * 0.5 * (sigma(pl) + sigma(pr))
+ *   x faces: dx_v = 0.5 (gx[i] + gx[i+1]) / dl (longitudinal slopes of the
+ *            two cells), dy_v = 0.5 (gy[i+1][j+1] + gy[i+2][j+1]) / dl;
+ *            F += (0, -mu (dx_vx - dy_vy), -mu (dx_vy + dy_vx))
+ *   y faces: dy_v = 0.5 (gy[i+2][j] + gy[i+2][j+1]) / dl,
+ *            dx_v = 0.5 (gx[i+1][j+1] + gx[i+1][j+2]) / dl;
+ *            F += (0, -mu (dx_vy + dy_vx), -(-mu (dx_vx - dy_vy)))
+ * (adjacent_mean, src/core_ndarray_ops.hpp:84-90: 0.5 * (a + b)). */
+static void update2d_fluxes_visc(const double* pe, int ni, int nj, int nc, double theta,
+                                 face_flux_fn flux, void* ctx, const double* visc, double* dfx, double* dfy);
+
 static void update2d_fluxes(const double* pe, int ni, int nj, int nc, double theta,
                             face_flux_fn flux, void* ctx, double* dfx, double* dfy)
+{
+    update2d_fluxes_visc(pe, ni, nj, nc, theta, flux, ctx, NULL, dfx, dfy);
+}
+
+static void update2d_fluxes_visc(const double* pe, int ni, int nj, int nc, double theta,
+                                 face_flux_fn flux, void* ctx, const double* visc, double* dfx, double* dfy)
 {
     const int ei = ni + 4, ej = nj + 4;
 #define PE(i, j, k) pe[((long)(i) * ej + (j)) * nc + (k)]
@@ -307,7 +329,25 @@ static void update2d_fluxes(const double* pe, int ni, int nj, int nc, double the
                 pl[k] = PE(i + 1, j + 2, k) + 0.5 * gx[((long)i * ej + (j + 2)) * nc + k];
                 pr[k] = PE(i + 2, j + 2, k) - 0.5 * gx[((long)(i + 1) * ej + (j + 2)) * nc + k];
             }
-            flux(ctx, pl, pr, 1, i, j, &fx[((long)i * nj + j) * nc]);
+            double* f = &fx[((long)i * nj + j) * nc];
+            flux(ctx, pl, pr, 1, i, j, f);
+            if (visc) {
+                const double nu = visc[0], dl = visc[1];
+                const double* ga = &gx[((long)i * ej + (j + 2)) * nc];
+                const double* gb = &gx[((long)(i + 1) * ej + (j + 2)) * nc];
+                const double* ta = &gy[((long)(i + 1) * (nj + 2) + (j + 1)) * nc];
+                const double* tb = &gy[((long)(i + 2) * (nj + 2) + (j + 1)) * nc];
+                double mu = nu * 0.5 * (pr[0] + pl[0]);
+                double dx_vx = 0.5 * (ga[1] + gb[1]) / dl;
+                double dy_vx = 0.5 * (ta[1] + tb[1]) / dl;
+                double dx_vy = 0.5 * (ga[2] + gb[2]) / dl;
+                double dy_vy = 0.5 * (ta[2] + tb[2]) / dl;
+                double tauxx = mu * (dx_vx - dy_vy);
+                double tauxy = mu * (dx_vy + dy_vx);
+                f[0] = f[0] + 0.0;
+                f[1] = f[1] + -tauxx;
+                f[2] = f[2] + -tauxy;
+            }
         }
     for (int i = 0; i < ni; ++i)
         for (int j = 0; j < nj + 1; ++j) {
@@ -315,7 +355,25 @@ static void update2d_fluxes(const double* pe, int ni, int nj, int nc, double the
                 pl[k] = PE(i + 2, j + 1, k) + 0.5 * gy[((long)(i + 2) * (nj + 2) + j) * nc + k];
                 pr[k] = PE(i + 2, j + 2, k) - 0.5 * gy[((long)(i + 2) * (nj + 2) + (j + 1)) * nc + k];
             }
-            flux(ctx, pl, pr, 2, i, j, &fy[((long)i * (nj + 1) + j) * nc]);
+            double* f = &fy[((long)i * (nj + 1) + j) * nc];
+            flux(ctx, pl, pr, 2, i, j, f);
+            if (visc) {
+                const double nu = visc[0], dl = visc[1];
+                const double* ga = &gy[((long)(i + 2) * (nj + 2) + j) * nc];
+                const double* gb = &gy[((long)(i + 2) * (nj + 2) + (j + 1)) * nc];
+                const double* ta = &gx[((long)(i + 1) * ej + (j + 1)) * nc];
+                const double* tb = &gx[((long)(i + 1) * ej + (j + 2)) * nc];
+                double mu = nu * 0.5 * (pr[0] + pl[0]);
+                double dx_vx = 0.5 * (ta[1] + tb[1]) / dl;
+                double dy_vx = 0.5 * (ga[1] + gb[1]) / dl;
+                double dx_vy = 0.5 * (ta[2] + tb[2]) / dl;
+                double dy_vy = 0.5 * (ga[2] + gb[2]) / dl;
+                double tauyx = mu * (dx_vy + dy_vx);
+                double tauyy = -mu * (dx_vx - dy_vy);
+                f[0] = f[0] + 0.0;
+                f[1] = f[1] + -tauyx;
+                f[2] = f[2] + -tauyy;
+            }
         }
     for (int i = 0; i < ni; ++i)
         for (int j = 0; j < nj; ++j)
@@ -390,9 +448,9 @@ int orc_euler1d_updated_conserved(const double* uc, const double* pe, int n,
 }
 
 /* ------------------------------------------------------------------------ */
-/* locally isothermal: src/module_locally_isothermal.cpp:182-212 with zero
- * viscosity (godunov part :115-122,171-174); the temperature mapping is
- * supplied as face arrays cs2_x[(ni+1)][nj], cs2_y[ni][(nj+1)] evaluated at
+/* locally isothermal: src/module_locally_isothermal.cpp:182-212 (godunov
+ * and viscous fluxes :106-174); the temperature mapping is supplied as face
+ * arrays cs2_x[(ni+1)][nj], cs2_y[ni][(nj+1)] evaluated at
  * mesh_geometry.face_coordinates (the mapping is a host std::function). */
 typedef struct { const double* cs2x; const double* cs2y; int nj; } iso_face_ctx;
 
@@ -404,13 +462,14 @@ static void iso_face_table(void* c, const double* pl, const double* pr, int axis
 }
 
 int orc_locally_isothermal_updated_conserved(const double* uc, const double* pe, int ni, int nj,
-                                  double dt, double dl, const double* cs2_x, const double* cs2_y, double* out)
+                                  double dt, double dl, const double* cs2_x, const double* cs2_y, double nu, double* out)
 {
     long nz = (long)ni * nj;
     double* dfx = (double*)malloc(sizeof(double) * nz * 3);
     double* dfy = (double*)malloc(sizeof(double) * nz * 3);
     iso_face_ctx ctx = { cs2_x, cs2_y, nj };
-    update2d_fluxes(pe, ni, nj, 3, 2.0, iso_face_table, &ctx, dfx, dfy);
+    const double visc[2] = { nu, dl };
+    update2d_fluxes_visc(pe, ni, nj, 3, 2.0, iso_face_table, &ctx, nu == 0.0 ? NULL : visc, dfx, dfy);
     for (long z = 0; z < nz * 3; ++z)
         out[z] = uc[z] - (dfx[z] + dfy[z]) * dt / dl;
     free(dfx); free(dfy);
@@ -484,7 +543,8 @@ static void md_face(void* c, const double* pl, const double* pr, int axis, int i
     orc_iso2d_riemann_hlle(pl, pr, md_cs2(x, y, m->rs, m->mach), axis, f);
 }
 
-/* updated_conserved :294-355 (zero viscosity; theta = 2 hard-coded at :313-314) */
+/* updated_conserved :294-355 (theta = 2 hard-coded at :313-314; viscous
+ * fluxes :243-287 when kinematic_viscosity != 0) */
 int orc_minidisk_updated_conserved(const double* uc, const double* pe, const orc_minidisk_t* sd,
                                    int level, long bi, long bj, double time, double dt, double* out)
 {
@@ -496,12 +556,13 @@ int orc_minidisk_updated_conserved(const double* uc, const double* pe, const orc
     double* yc = (double*)malloc(sizeof(double) * n);
     double* ui = (double*)malloc(sizeof(double) * nz * nc);
     md_face_ctx ctx = { sd->domain_radius, sd->softening_length, sd->mach_number, n, level, bi, bj };
-    update2d_fluxes(pe, n, n, nc, 2.0, md_face, &ctx, dfx, dfy);
+    /* cell_size :78-81 */
+    double dl = 2.0 * sd->domain_radius / (double)sd->block_size / (double)(1 << level);
+    const double visc[2] = { sd->kinematic_viscosity, dl };
+    update2d_fluxes_visc(pe, n, n, nc, 2.0, md_face, &ctx, sd->kinematic_viscosity == 0.0 ? NULL : visc, dfx, dfy);
     orc_minidisk_cell_coordinates(sd, level, bi, bj, xc, yc);
     orc_minidisk_initial_conserved(sd, level, bi, bj, ui);
 
-    /* cell_size :78-81 */
-    double dl = 2.0 * sd->domain_radius / (double)sd->block_size / (double)(1 << level);
     double st[10];
     orc_two_body_state(1.0, 1.0, sd->mass_ratio, sd->eccentricity, time, sd->omega_frame, st);
     double Om = sd->omega_frame;
@@ -550,7 +611,12 @@ int orc_minidisk_updated_conserved(const double* uc, const double* pe, const orc
 
 /* ------------------------------------------------------------------------ */
 /* Guard fill: amr::extend_block (src/mesh_amr.hpp:334-393) over same-level
- * neighbours with the periodic next_on/prev_on (src/mesh_block_index.hpp:334-335). */
+ * neighbours with the periodic next_on/prev_on (src/mesh_block_index.hpp:334-335).
+ * Corner blocks as the reference assembles them (:361-379, and the same in
+ * problems/minidisk.cpp:87-120): c02 = prev_on(next_on(b, 0), 1) fills the
+ * (x-prev, y-next) corner and c20 = next_on(prev_on(b, 0), 1) the (x-next,
+ * y-prev) corner, i.e. those two corners come from the diagonally opposite
+ * block.  Only viscous updates read corner zones. */
 static inline long wrapb(long b, int nb) { return (b + nb) % nb; }
 
 void orc_extend_block_2d(const double* P, int nb, int n, int nc, long bi, long bj, double* pe)
@@ -562,7 +628,9 @@ void orc_extend_block_2d(const double* P, int nb, int n, int nc, long bi, long b
             int cj = j < 2 ? 0 : (j >= n + 2 ? 2 : 1);
             int ii = i < 2 ? i - 2 + n : (i >= n + 2 ? i - 2 - n : i - 2);
             int jj = j < 2 ? j - 2 + n : (j >= n + 2 ? j - 2 - n : j - 2);
-            long sb = wrapb(bi + ci - 1, nb) * nb + wrapb(bj + cj - 1, nb);
+            int oi = ci - 1, oj = cj - 1;
+            if (oi != 0 && oj != 0 && oi != oj) { oi = -oi; oj = -oj; }   /* c02 / c20 */
+            long sb = wrapb(bi + oi, nb) * nb + wrapb(bj + oj, nb);
             memcpy(&pe[((long)i * e + j) * nc], &P[((sb * n + ii) * (long)n + jj) * nc], sizeof(double) * nc);
         }
 }

@@ -72,7 +72,7 @@ int orc_euler1d_updated_conserved(const double* uc, const double* pe, int n,
 int orc_euler2d_updated_conserved(const double* uc, const double* pe, int ni, int nj,
                                   double dt, double dl, double theta, double gamma, int hllc, double* out);
 int orc_locally_isothermal_updated_conserved(const double* uc, const double* pe, int ni, int nj,
-                                  double dt, double dl, const double* cs2_x, const double* cs2_y, double* out);
+                                  double dt, double dl, const double* cs2_x, const double* cs2_y, double nu, double* out);
 int orc_minidisk_updated_conserved(const double* uc, const double* pe, const orc_minidisk_t* sd,
                                   int level, long bi, long bj, double time, double dt, double* out);
 void orc_minidisk_initial_conserved(const orc_minidisk_t* sd, int level, long bi, long bj, double* out);

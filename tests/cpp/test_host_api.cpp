@@ -117,7 +117,7 @@ static void test_locally_isothermal(std::mt19937& rng)
         for (std::size_t j = 0; j <= n; ++j) cy[i * (n + 1) + j] = temperature(geom.face_coordinate(block, 1, i, j));
     std::vector<double> ref(n * n * 3);
     orc_locally_isothermal_updated_conserved(flat(uc).data(), flat(pe).data(), int(n), int(n), 1e-3,
-                                             geom.cell_spacing(block), cx.data(), cy.data(), ref.data());
+                                             geom.cell_spacing(block), cx.data(), cy.data(), 0.0, ref.data());
     EXPECT(flat(out) == ref, "locally_isothermal::updated_conserved equals the oracle exactly");
 }
 

@@ -127,6 +127,29 @@ def minidisk_cases(rng):
     return dict(sd=np.array(sds), uc=np.array(ucs), pe=np.array(pes), out=np.array(outs), meta=np.array(meta))
 
 
+def minidisk_visc_cases(rng):
+    """minidisk::updated_conserved with a nonzero kinematic viscosity (the
+    viscous stress of godunov_and_viscous_fluxes, problems/minidisk_scheme.cpp:243-287)."""
+    n = 16
+    sds, ucs, pes, outs, meta = [], [], [], [], []
+    for level, bi, bj, time, dt, nu in ((2, 1, 2, 0.4, 1e-3, 1e-3), (1, 0, 1, 1.1, 5e-4, 0.05), (3, 5, 2, 0.0, 1e-3, 0.2)):
+        sd = np.array([n, level, 1.0, 10.0, nu, 1.5, 0.05, 0.0, 1.0, 0.05, 1e3, 1e2, 0.2, 0.3])
+        ui = np.zeros((n, n, 3))
+        oracle.ref().ref_minidisk_initial_conserved(P(sd), level, bi, bj, P(ui))
+        uc = np.ascontiguousarray(ui * np.exp(0.05 * rng.normal(size=(n, n, 1))))
+        pe = np.zeros((n + 4, n + 4, 3))
+        pe[..., 0] = rng.uniform(0.5, 1.5, (n + 4, n + 4))
+        pe[..., 1:] = rng.normal(0, 1, (n + 4, n + 4, 2))
+        out = np.zeros((n, n, 3))
+        oracle.ref().ref_minidisk_updated_conserved(P(uc), P(pe), P(sd), level, bi, bj, time, dt, P(out))
+        sds.append(sd)
+        ucs.append(uc)
+        pes.append(pe)
+        outs.append(out)
+        meta.append((level, bi, bj, time, dt))
+    return dict(sd=np.array(sds), uc=np.array(ucs), pe=np.array(pes), out=np.array(outs), meta=np.array(meta))
+
+
 def mhd_cases(rng):
     """mhd::riemann_hlle vectors and one advance_mhd_ct call on an 8^3 periodic cube."""
     m = 300
@@ -182,6 +205,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "minidisk_block.npz"), **minidisk_cases(rng))
     np.savez_compressed(os.path.join(HERE, "mhd_ct.npz"), **mhd_cases(rng))
     np.savez_compressed(os.path.join(HERE, "srhd_pointwise.npz"), **srhd_cases(rng))
+    np.savez_compressed(os.path.join(HERE, "minidisk_visc_block.npz"), **minidisk_visc_cases(rng))
     print("golden vectors written to", HERE)
 
 

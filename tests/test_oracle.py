@@ -84,6 +84,67 @@ def test_minidisk_block_golden():
         assert np.array_equal(got, out)
 
 
+def test_minidisk_viscous_block_golden():
+    """The viscous stress (problems/minidisk_scheme.cpp:243-287) at nu in
+    {1e-3, 0.05, 0.2}: golden outputs of the compiled reference."""
+    g = golden("minidisk_visc_block.npz")
+    for sd, uc, pe, out, (level, bi, bj, time, dt) in zip(g["sd"], g["uc"], g["pe"], g["out"], g["meta"]):
+        assert sd[4] > 0
+        s = _sd_from_array(sd)
+        got = np.zeros_like(out)
+        orc().orc_minidisk_updated_conserved(P(c(uc)), P(c(pe)), ctypes.byref(s), int(level), int(bi), int(bj),
+                                             time, dt, P(got))
+        assert np.array_equal(got, out)
+        # the stress changes the result (the fixture is not the inviscid one)
+        s0 = _sd_from_array(np.concatenate([sd[:4], [0.0], sd[5:]]))
+        inviscid = np.zeros_like(out)
+        orc().orc_minidisk_updated_conserved(P(c(uc)), P(c(pe)), ctypes.byref(s0), int(level), int(bi), int(bj),
+                                             time, dt, P(inviscid))
+        assert not np.array_equal(inviscid, out)
+
+
+def test_locally_isothermal_viscous_matches_minidisk_fluxes():
+    """locally_isothermal and minidisk share godunov_and_viscous_fluxes
+    (src/module_locally_isothermal.cpp:106-174, problems/minidisk_scheme.cpp:220-287):
+    with the minidisk face sound speeds as tables and the same nu, the change
+    that the viscous stress makes to the update is the same in both."""
+    n, level, bi, bj, nu = 16, 2, 1, 2, 0.03
+    rng = np.random.default_rng(4)
+    sd = np.array([n, level, 1.0, 10.0, nu, 1.5, 0.05, 0.0, 1.0, 0.05, 1e3, 1e2, 0.2, 0.3])
+    uc = c(np.abs(rng.normal(1, 0.1, (n, n, 3))))
+    pe = c(np.concatenate([rng.uniform(0.5, 1.5, (n + 4, n + 4, 1)), rng.normal(0, 1, (n + 4, n + 4, 2))], -1))
+    dl = 2.0 * 1.5 / n / (1 << level)
+    R = 1.5
+
+    def vert(b, k):
+        x0 = R * (-1 + (2 * b) / float(1 << level))
+        x1 = R * (-1 + (2 * (b + 1)) / float(1 << level))
+        return x0 + (x1 - x0) * k / float(n)
+    xv = np.array([vert(bi, k) for k in range(n + 1)])
+    yv = np.array([vert(bj, k) for k in range(n + 1)])
+    xc, yc = 0.5 * (xv[:-1] + xv[1:]), 0.5 * (yv[:-1] + yv[1:])
+    cs2 = lambda x, y: -(-1.0 * 1.0 / np.sqrt(x * x + y * y + 0.05 ** 2.0)) / 10.0 / 10.0
+    cx = c(cs2(xv[:, None], yc[None, :]))
+    cy = c(cs2(xc[:, None], yv[None, :]))
+
+    def minidisk(nu_):
+        sd_ = sd.copy()
+        sd_[4] = nu_
+        out = np.zeros((n, n, 3))
+        orc().orc_minidisk_updated_conserved(P(uc), P(pe), ctypes.byref(_sd_from_array(sd_)), level, bi, bj,
+                                             0.0, 1e-3, P(out))
+        return out
+
+    def iso(nu_):
+        out = np.zeros((n, n, 3))
+        orc().orc_locally_isothermal_updated_conserved(P(uc), P(pe), n, n, 1e-3, dl, P(cx), P(cy), nu_, P(out))
+        return out
+    dm = minidisk(nu) - minidisk(0.0)
+    di = iso(nu) - iso(0.0)
+    assert np.abs(dm).max() > 1e-6
+    assert np.allclose(dm, di, rtol=1e-9, atol=1e-14)
+
+
 # --------------------------------------------------------------------------- live reference
 @needs_ref
 def test_minidisk_live_reference_random_blocks():
@@ -95,6 +156,11 @@ def test_minidisk_live_reference_random_blocks():
             uc = c(np.abs(rng.normal(1, 0.1, (n, n, 3))))
             pe = c(np.concatenate([rng.uniform(0.5, 1.5, (n + 4, n + 4, 1)), rng.normal(0, 1, (n + 4, n + 4, 2))], -1))
             a, b = np.zeros((n, n, 3)), np.zeros((n, n, 3))
+            oracle.ref().ref_minidisk_updated_conserved(P(uc), P(pe), P(sd), level, bi, bj, 0.3, 1e-3, P(a))
+            orc().orc_minidisk_updated_conserved(P(uc), P(pe), ctypes.byref(_sd_from_array(sd)), level, bi, bj,
+                                                 0.3, 1e-3, P(b))
+            assert np.array_equal(a, b)
+            sd[4] = 0.02     # viscous
             oracle.ref().ref_minidisk_updated_conserved(P(uc), P(pe), P(sd), level, bi, bj, 0.3, 1e-3, P(a))
             orc().orc_minidisk_updated_conserved(P(uc), P(pe), ctypes.byref(_sd_from_array(sd)), level, bi, bj,
                                                  0.3, 1e-3, P(b))
@@ -245,7 +311,9 @@ def test_iso2d_consistency():
 
 def test_extend_block_2d_brute_force():
     """amr::extend_block over a periodic level (src/mesh_amr.hpp:360-393): every guard
-    zone holds the zone of the wrapped global index."""
+    zone holds the zone of the wrapped global index, except the two
+    off-diagonal corners, which the reference takes from the opposite
+    diagonal neighbour."""
     nb, n, nc = 4, 4, 2
     N = nb * n
     glob = np.arange(N * N * nc, dtype=np.float64).reshape(N, N, nc)
@@ -256,7 +324,17 @@ def test_extend_block_2d_brute_force():
             orc().orc_extend_block_2d(P(blocks), nb, n, nc, bi, bj, P(pe))
             gi = (bi * n - 2 + np.arange(n + 4)) % N
             gj = (bj * n - 2 + np.arange(n + 4)) % N
-            assert np.array_equal(pe, glob[np.ix_(gi, gj)])
+            want = glob[np.ix_(gi, gj)].copy()
+            # the reference's (x-prev, y-next) and (x-next, y-prev) corners come from
+            # the opposite diagonal block, at the same local indices (mesh_amr.hpp:361-379)
+            lo, hi = slice(0, 2), slice(n + 2, n + 4)
+            want[lo, hi] = glob[np.ix_((gi[lo] + 2 * n) % N, (gj[hi] - 2 * n) % N)]
+            want[hi, lo] = glob[np.ix_((gi[hi] - 2 * n) % N, (gj[lo] + 2 * n) % N)]
+            assert np.array_equal(pe, want)
+            # everything but those two corners is the wrapped global field
+            mask = np.ones((n + 4, n + 4), bool)
+            mask[lo, hi] = mask[hi, lo] = False
+            assert np.array_equal(pe[mask], glob[np.ix_(gi, gj)][mask])
 
 
 def _kh_state(nb, n, seed=1):
