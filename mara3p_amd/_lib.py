@@ -8,7 +8,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY_PATH = os.path.join(_HERE, "libmara3p_amd.so")
+# The in-tree build; MARA3P_AMD_LIBRARY may name another build of the same library.
+LIBRARY_PATH = os.environ.get("MARA3P_AMD_LIBRARY", os.path.join(_HERE, "libmara3p_amd.so"))
 
 M3P_OK = 0
 M3P_ERR_NEGATIVE_DENSITY = 1

@@ -48,14 +48,21 @@ struct MeshLoader {
     // neighbour.  Only the viscous kernels read corners.
     __device__ __forceinline__ const double* zone(int b, int r, int c) const
     {
-        if (r < 0 || r >= n) {
-            const bool lo = r < 0;
-            r += lo ? n : -n;
-            if (c < 0)       { b = lo ? nbr[4 * bxm + 2] : nbr[4 * bxm + 3]; c += n; }
-            else if (c >= n) { b = lo ? nbr[4 * bxp + 2] : nbr[4 * bxp + 3]; c -= n; }
-            else b = lo ? bxm : bxp;
-        } else if (c < 0)  { b = bym; c += n; }
-        else if (c >= n)   { b = byp; c -= n; }
+        // The four neighbour indices are copied to locals first: selecting
+        // between the loader's own fields keeps the loader in scratch memory.
+        const int xm = bxm, xp = bxp, ym = bym, yp = byp;
+        if (r < 0) {
+            r += n;
+            if (c < 0)       { b = nbr[4 * xm + 2]; c += n; }
+            else if (c >= n) { b = nbr[4 * xp + 2]; c -= n; }
+            else b = xm;
+        } else if (r >= n) {
+            r -= n;
+            if (c < 0)       { b = nbr[4 * xm + 3]; c += n; }
+            else if (c >= n) { b = nbr[4 * xp + 3]; c -= n; }
+            else b = xp;
+        } else if (c < 0)  { b = ym; c += n; }
+        else if (c >= n)   { b = yp; c -= n; }
         return u + ((size_t)b * S::NC * n + r) * n + c;
     }
     // Start of row r of the bound block b (wave-uniform); columns must lie in [0, n).
