@@ -199,6 +199,7 @@ MinidiskParams minidisk_params(const m3p_minidisk_data* d, double time)
     p.rs = d->softening_length;
     p.s2 = std::pow(d->softening_length, 2.0);
     p.mach = d->mach_number;
+    p.inv_mach2 = 1.0 / (d->mach_number * d->mach_number);
     p.omega = d->omega_frame;
     p.sink_rate = d->sink_rate;
     p.s6 = std::pow(d->sink_radius, 6.0);
@@ -319,18 +320,13 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
     tail += (256 - (reinterpret_cast<uintptr_t>(tail) & 255)) & 255;
     a.vmax = reinterpret_cast<unsigned long long*>(tail);
     if (!a.dt) a.dt = reinterpret_cast<double*>(tail + 8);
-    const int gc = (int)((N3 + 255) / 256), gf = (int)((nf + 255) / 256);
+    const int gc = (int)((N3 + 255) / 256);
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, a.vmax, 1, 0ull);
-    hipLaunchKernelGGL(mhd_prim_kernel<R>, dim3(gc), dim3(256), 0, s, a, P);
+    hipLaunchKernelGGL(mhd_prim_kernel<R>, dim3(std::min(gc, 4096)), dim3(256), 0, s, a, P);
     hipLaunchKernelGGL(mhd_dt_kernel, dim3(1), dim3(1), 0, s, a);
-    hipLaunchKernelGGL((mhd_face_kernel<R, 1>), dim3(gf), dim3(256), 0, s, a, (const R*)P, F1, E1);
-    hipLaunchKernelGGL((mhd_face_kernel<R, 2>), dim3(gf), dim3(256), 0, s, a, (const R*)P, F2, E2);
-    hipLaunchKernelGGL((mhd_face_kernel<R, 3>), dim3(gf), dim3(256), 0, s, a, (const R*)P, F3, E3);
-    hipLaunchKernelGGL(mhd_cell_update_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)F1, (const R*)F2, (const R*)F3);
+    hipLaunchKernelGGL(mhd_faces_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)P, F1, E1, F2, E2, F3, E3);
     FaceE<R> fe{ E1, E2, E3, N, nf };
-    hipLaunchKernelGGL((mhd_face_update_kernel<R, 1>), dim3(gf), dim3(256), 0, s, a, fe);
-    hipLaunchKernelGGL((mhd_face_update_kernel<R, 2>), dim3(gf), dim3(256), 0, s, a, fe);
-    hipLaunchKernelGGL((mhd_face_update_kernel<R, 3>), dim3(gf), dim3(256), 0, s, a, fe);
+    hipLaunchKernelGGL(mhd_update_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)F1, (const R*)F2, (const R*)F3, fe);
     return launch_status("m3p_mhd_advance_ct");
 }
 
@@ -610,7 +606,7 @@ int m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, i
     const long N3 = (long)N * N * N;
     unsigned long long* slot = ds->red + 3000;
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, slot, 1, 0ull);
-    hipLaunchKernelGGL(mhd_divb_kernel, dim3((N3 + 255) / 256), dim3(256), 0, s, bf1, bf2, bf3, N, slot);
+    hipLaunchKernelGGL(mhd_divb_kernel, dim3((int)std::min<long>((N3 + 255) / 256, 4096)), dim3(256), 0, s, bf1, bf2, bf3, N, slot);
     hipLaunchKernelGGL(copy_u64_as_double_kernel, dim3(1), dim3(1), 0, s, slot, out);
     return launch_status("m3p_mhd_max_div_b");
 }

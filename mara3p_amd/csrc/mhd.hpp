@@ -1,17 +1,19 @@
 // Ideal MHD with constrained transport on a periodic N^3 cube:
-// mara::advance_mhd_ct (src/scheme_mhd_v1.cpp:265-299) as four kernels.
+// mara::advance_mhd_ct (src/scheme_mhd_v1.cpp:265-299) as three passes.
 //
 //   1. mhd_prim_kernel     cell primitives from U and the face-averaged B
 //                          (primitive_array :214-224, face_to_cell
 //                          src/mesh_cartesian_3d.hpp:235-245) + CFL max speed
-//   2. mhd_face_kernel<A>  PLM face states along axis A, HLLE Godunov flux and
-//                          face electric field (godunov_fluxes_plm :152-179,
+//   2. mhd_faces_kernel    PLM face states, HLLE Godunov flux and face electric
+//                          field of the three faces below each cell
+//                          (godunov_fluxes_plm :152-179,
 //                          mhd::riemann_hlle src/physics_mhd.hpp:358-390)
-//   3. mhd_cell_update     U - div(F) dt / dl (divergence_difference
-//                          src/mesh_cartesian_3d.hpp:290-301)
-//   4. mhd_face_update<A>  B_f - curl(E) dt / dl with edge EMFs averaged from
-//                          the faces (edge_emf_from_face :76-98,
-//                          solenoidal_difference src/mesh_cartesian_3d.hpp:265-276)
+//   3. mhd_update_kernel   U - div(F) dt / dl (divergence_difference
+//                          src/mesh_cartesian_3d.hpp:290-301) and, for the
+//                          faces below each cell, B_f - curl(E) dt / dl with
+//                          edge EMFs averaged from the faces
+//                          (edge_emf_from_face :76-98, solenoidal_difference
+//                          src/mesh_cartesian_3d.hpp:265-276)
 //
 // Layouts (row-major, last axis fastest, as the reference arrays):
 //   U [5][N][N][N];  B1 [N+1][N][N], B2 [N][N+1][N], B3 [N][N][N+1]
@@ -208,6 +210,9 @@ struct MhdArgs {
 
 __device__ __forceinline__ int wrapi(int i, int N) { return i < 0 ? i + N : (i >= N ? i - N : i); }
 
+// Grid-stride over cells with a bounded grid, so the speed reduction ends in
+// one atomic per workgroup of a few thousand (one per 256 cells would
+// serialise on the single slot).
 template<typename R>
 __global__ void __launch_bounds__(256)
 mhd_prim_kernel(MhdArgs a, R* P)
@@ -215,11 +220,10 @@ mhd_prim_kernel(MhdArgs a, R* P)
     __shared__ double part[4];
     const int N = a.N;
     const long N3 = (long)N * N * N;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long NN = (long)N * N;
     double s = 0.0;
-    if (c < N3) {
-        const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);
-        const long NN = (long)N * N;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < N3; c += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(c / NN), j = (int)(c / N % N), k = (int)(c % N);
         double b[3] = { 0.5 * (a.b1[(long)i * NN + (long)j * N + k] + a.b1[(long)(i + 1) * NN + (long)j * N + k]),
                         0.5 * (a.b2[((long)i * (N + 1) + j) * N + k] + a.b2[((long)i * (N + 1) + j + 1) * N + k]),
                         0.5 * (a.b3[((long)i * N + j) * (N + 1) + k] + a.b3[((long)i * N + j) * (N + 1) + k + 1]) };
@@ -230,7 +234,7 @@ mhd_prim_kernel(MhdArgs a, R* P)
         if (st) report_error(a.err, st == 1 ? 1 : 2, 0, (int)c);
 #pragma unroll
         for (int q = 0; q < 8; ++q) P[q * N3 + c] = R(p[q]);
-        s = mhd_fastest_speed(p, a.gamma);
+        s = fmax(s, mhd_fastest_speed(p, a.gamma));
     }
     s = wave_max(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
@@ -248,21 +252,23 @@ __global__ void mhd_dt_kernel(MhdArgs a)
     *a.dt = a.cfl * a.dl / vmax;
 }
 
-// Faces of axis A: index (i, j, k) with N+1 entries along A.  Cells along the
-// axis are (m-2, m-1, m, m+1) for face m, wrapped periodically.
+// All three face families in one pass: the thread of cell (i, j, k)
+// computes the lower face of the cell on each axis (the x face at i, the y
+// face at j, the z face at k), and the cells on the last plane of an axis
+// also compute the face at N (periodic: the same cells as face 0).  The
+// primitive stencils of the three axes share the cell's neighbourhood, so
+// the primitives are read once from cache instead of once per axis.  Each
+// face's stencil: cells (m-2, m-1, m, m+1) along the axis for face m, wrapped.
 template<typename R, int A>
-__global__ void __launch_bounds__(256)
-mhd_face_kernel(MhdArgs a, const R* P, R* F, R* E)
+__device__ __forceinline__ void mhd_one_face(const MhdArgs& a, const R* P, R* F, R* E, int i, int j, int k)
 {
     const int N = a.N;
     const long N3 = (long)N * N * N;
     const long nf = (long)(N + 1) * N * N;
-    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
-    int i, j, k;
-    if (A == 1)      { i = (int)(f / ((long)N * N)); j = (int)(f / N % N); k = (int)(f % N); }
-    else if (A == 2) { i = (int)(f / ((long)(N + 1) * N)); j = (int)(f / N % (N + 1)); k = (int)(f % N); }
-    else             { i = (int)(f / ((long)N * (N + 1))); j = (int)(f / (N + 1) % N); k = (int)(f % (N + 1)); }
+    long f;
+    if (A == 1)      f = ((long)i * N + j) * N + k;
+    else if (A == 2) f = ((long)i * (N + 1) + j) * N + k;
+    else             f = ((long)i * N + j) * (N + 1) + k;
     long cell[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -286,9 +292,25 @@ mhd_face_kernel(MhdArgs a, const R* P, R* F, R* E)
     Mhd<R>::template riemann<A>(pl, pr, R(bf[f]), R(a.gamma), fl, e);
 #pragma unroll
     for (int q = 0; q < 5; ++q) F[q * nf + f] = fl[q];
-    // keep the two transverse components: A=1 -> (Ey, Ez), A=2 -> (Ex, Ez), A=3 -> (Ex, Ey)
     E[f] = e[A == 1 ? 1 : 0];
     E[nf + f] = e[A == 3 ? 1 : 2];
+}
+
+template<typename R>
+__global__ void __launch_bounds__(256)
+mhd_faces_kernel(MhdArgs a, const R* P, R* F1, R* E1, R* F2, R* E2, R* F3, R* E3)
+{
+    const int N = a.N;
+    const long N3 = (long)N * N * N;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N3) return;
+    const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);
+    mhd_one_face<R, 1>(a, P, F1, E1, i, j, k);
+    mhd_one_face<R, 2>(a, P, F2, E2, i, j, k);
+    mhd_one_face<R, 3>(a, P, F3, E3, i, j, k);
+    if (i == N - 1) mhd_one_face<R, 1>(a, P, F1, E1, N, j, k);
+    if (j == N - 1) mhd_one_face<R, 2>(a, P, F2, E2, i, N, k);
+    if (k == N - 1) mhd_one_face<R, 3>(a, P, F3, E3, i, j, N);
 }
 
 // Face-centred transverse electric field of axis-A faces, by component.
@@ -333,52 +355,28 @@ struct FaceE {
     }
 };
 
-template<typename R>
-__global__ void __launch_bounds__(256)
-mhd_cell_update_kernel(MhdArgs a, const R* F1, const R* F2, const R* F3)
-{
-    const int N = a.N;
-    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N3) return;
-    const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);
-    const long x0 = ((long)i * N + j) * N + k, x1 = x0 + (long)N * N;
-    const long y0 = ((long)i * (N + 1) + j) * N + k, y1 = y0 + N;
-    const long z0 = ((long)i * N + j) * (N + 1) + k, z1 = z0 + 1;
-    const double dt = *a.dt;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        double d1 = double(F1[q * nf + x1]) - double(F1[q * nf + x0]);
-        double d2 = double(F2[q * nf + y1]) - double(F2[q * nf + y0]);
-        double d3 = double(F3[q * nf + z1]) - double(F3[q * nf + z0]);
-        double r = a.uc[q * N3 + c] - (d1 + d2 + d3) * dt / a.dl;
-        if (a.u0) r = a.u0[q * N3 + c] * a.rk_b + r * (1.0 - a.rk_b);
-        a.uo[q * N3 + c] = r;
-    }
-}
-
+// One pass for the whole update: the thread of cell (i, j, k) updates the
+// cell's conserved state and the lower face of the cell on each axis (plus
+// the faces at N on the last planes), with the arithmetic of
+// mhd_cell_update_kernel and mhd_face_update_kernel<R, A>.
 template<typename R, int A>
-__global__ void __launch_bounds__(256)
-mhd_face_update_kernel(MhdArgs a, FaceE<R> fe)
+__device__ __forceinline__ void mhd_update_face(const MhdArgs& a, const FaceE<R>& fe, double dt, int i, int j, int k)
 {
     const int N = a.N;
-    const long nf = (long)(N + 1) * N * N;
-    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
-    const double dt = *a.dt;
     double cf, b, b0 = 0.0;
+    long f;
     if (A == 1) {
-        const int i = (int)(f / ((long)N * N)), j = (int)(f / N % N), k = (int)(f % N);
+        f = ((long)i * N + j) * N + k;
         cf = (fe.ee3(i, j + 1, k) - fe.ee3(i, j, k)) - (fe.ee2(i, j, k + 1) - fe.ee2(i, j, k));
         b = a.b1[f];
         if (a.u0) b0 = a.b01[f];
     } else if (A == 2) {
-        const int i = (int)(f / ((long)(N + 1) * N)), j = (int)(f / N % (N + 1)), k = (int)(f % N);
+        f = ((long)i * (N + 1) + j) * N + k;
         cf = (fe.ee1(i, j, k + 1) - fe.ee1(i, j, k)) - (fe.ee3(i + 1, j, k) - fe.ee3(i, j, k));
         b = a.b2[f];
         if (a.u0) b0 = a.b02[f];
     } else {
-        const int i = (int)(f / ((long)N * (N + 1))), j = (int)(f / (N + 1) % N), k = (int)(f % (N + 1));
+        f = ((long)i * N + j) * (N + 1) + k;
         cf = (fe.ee2(i + 1, j, k) - fe.ee2(i, j, k)) - (fe.ee1(i, j + 1, k) - fe.ee1(i, j, k));
         b = a.b3[f];
         if (a.u0) b0 = a.b03[f];
@@ -389,21 +387,50 @@ mhd_face_update_kernel(MhdArgs a, FaceE<R> fe)
     out[f] = r;
 }
 
+template<typename R>
+__global__ void __launch_bounds__(256)
+mhd_update_kernel(MhdArgs a, const R* F1, const R* F2, const R* F3, FaceE<R> fe)
+{
+    const int N = a.N;
+    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N3) return;
+    const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);
+    const double dt = *a.dt;
+    const long x0 = ((long)i * N + j) * N + k, x1 = x0 + (long)N * N;
+    const long y0 = ((long)i * (N + 1) + j) * N + k, y1 = y0 + N;
+    const long z0 = ((long)i * N + j) * (N + 1) + k, z1 = z0 + 1;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        double d1 = double(F1[q * nf + x1]) - double(F1[q * nf + x0]);
+        double d2 = double(F2[q * nf + y1]) - double(F2[q * nf + y0]);
+        double d3 = double(F3[q * nf + z1]) - double(F3[q * nf + z0]);
+        double r = a.uc[q * N3 + c] - (d1 + d2 + d3) * dt / a.dl;
+        if (a.u0) r = a.u0[q * N3 + c] * a.rk_b + r * (1.0 - a.rk_b);
+        a.uo[q * N3 + c] = r;
+    }
+    mhd_update_face<R, 1>(a, fe, dt, i, j, k);
+    mhd_update_face<R, 2>(a, fe, dt, i, j, k);
+    mhd_update_face<R, 3>(a, fe, dt, i, j, k);
+    if (i == N - 1) mhd_update_face<R, 1>(a, fe, dt, N, j, k);
+    if (j == N - 1) mhd_update_face<R, 2>(a, fe, dt, i, N, k);
+    if (k == N - 1) mhd_update_face<R, 3>(a, fe, dt, i, j, N);
+}
+
 // max over cells of |div B| = |d1(B1) + d2(B2) + d3(B3)| (divergence_difference)
 __global__ void __launch_bounds__(256)
 mhd_divb_kernel(const double* b1, const double* b2, const double* b3, int N, unsigned long long* slot)
 {
     __shared__ double part[4];
     const long N3 = (long)N * N * N;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long NN = (long)N * N;
     double v = 0.0;
-    if (c < N3) {
-        const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);
-        const long NN = (long)N * N;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < N3; c += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(c / NN), j = (int)(c / N % N), k = (int)(c % N);
         double d1 = b1[(long)(i + 1) * NN + (long)j * N + k] - b1[(long)i * NN + (long)j * N + k];
         double d2 = b2[((long)i * (N + 1) + j + 1) * N + k] - b2[((long)i * (N + 1) + j) * N + k];
         double d3 = b3[((long)i * N + j) * (N + 1) + k + 1] - b3[((long)i * N + j) * (N + 1) + k];
-        v = fabs(d1 + d2 + d3);
+        v = fmax(v, fabs(d1 + d2 + d3));
     }
     v = wave_max(v);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;

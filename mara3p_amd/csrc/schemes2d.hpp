@@ -67,6 +67,7 @@ struct MinidiskParams {
     double rs;           // softening length
     double s2;           // pow(rs, 2) evaluated on the host (std::pow)
     double mach;
+    double inv_mach2;    // 1 / mach^2
     double omega;        // frame rotation rate
     double sink_rate;
     double s6;           // pow(sink_radius, 6) evaluated on the host
@@ -155,9 +156,10 @@ struct MinidiskScheme {
                 x = 0.5 * (td.xv[fi - tc.k0] + td.xv[fi - tc.k0 + 1]);
                 y = td.yv[fj - tc.c0];
             }
+            // -phi / M^2 with phi = -1 / sqrt(r^2 + r_s^2); the division by
+            // M twice is folded into one product (within an ulp)
             double r2 = x * x + y * y;
-            double ph = -1.0 * 1.0 / sqrt(r2 + prm.s2);
-            cs2 = -ph / prm.mach / prm.mach;
+            cs2 = (1.0 / sqrt(r2 + prm.s2)) * prm.inv_mach2;
         } else {
             float xf = AXIS == 1 ? td.xvf[fi - tc.k0] : td.xcf[fi - tc.k0];
             float yf = AXIS == 1 ? td.ycf[fj - tc.c0] : td.yvf[fj - tc.c0];
@@ -177,45 +179,56 @@ struct MinidiskScheme {
         if constexpr (is_fp64<R>::value) {
             double x = 0.5 * (td.xv[k - tc.k0] + td.xv[k - tc.k0 + 1]);
             double y = 0.5 * (td.yv[c - tc.c0] + td.yv[c - tc.c0 + 1]);
-            // two_body::gravitational_acceleration (src/physics_two_body.hpp:208-218)
+            // two_body::gravitational_acceleration (src/physics_two_body.hpp:208-218);
+            // pow(q, 1.5) = q sqrt(q) and pow(t, 3) = t t t, within a few ulp
             double dx1 = x - prm.x1, dy1 = y - prm.y1;
             double dx2 = x - prm.x2, dy2 = y - prm.y2;
             double r21 = dx1 * dx1 + dy1 * dy1;
             double r22 = dx2 * dx2 + dy2 * dy2;
-            double g1 = -1.0 * prm.m1 / pow(r21 + prm.s2, 1.5);
-            double g2 = -1.0 * prm.m2 / pow(r22 + prm.s2, 1.5);
+            double q1 = r21 + prm.s2, q2 = r22 + prm.s2;
+            double g1 = -1.0 * prm.m1 / (q1 * sqrt(q1));
+            double g2 = -1.0 * prm.m2 / (q2 * sqrt(q2));
             double ax1 = g1 * dx1, ay1 = g1 * dy1;
             double ax2 = g2 * dx2, ay2 = g2 * dy2;
             // coriolis_term :67-71, centrifugal_term :73-76 (from pc)
             double om = prm.omega;
             double corx = 2.0 * om * pc[2], cory = 2.0 * om * (-pc[1]);
             double cenx = om * om * x, ceny = om * om * y;
-            // sink_rate_field :57-65
-            double r61 = pow((0.0 + dx1 * dx1) + dy1 * dy1, 3.0);
-            double r62 = pow((0.0 + dx2 * dx2) + dy2 * dy2, 3.0);
-            double sr1 = prm.sink_rate * exp(-r61 / prm.s6);
-            double sr2 = prm.sink_rate * exp(-r62 / prm.s6);
-            // buffer_rate_field :47-55
+            // sink_rate_field :57-65; exp(-t) is exactly 0 for t > 746
+            double t1 = (0.0 + dx1 * dx1) + dy1 * dy1;
+            double t2 = (0.0 + dx2 * dx2) + dy2 * dy2;
+            double r61 = t1 * t1 * t1, r62 = t2 * t2 * t2;
+            double sr1 = 0.0, sr2 = 0.0;
+            if (r61 < 746.0 * prm.s6) sr1 = prm.sink_rate * exp(-r61 / prm.s6);
+            if (r62 < 746.0 * prm.s6) sr2 = prm.sink_rate * exp(-r62 / prm.s6);
+            // buffer_rate_field :47-55; 1 + tanh(y) is exactly 0 for y < -20,
+            // and then the initial state (initial_conserved_array :182-191)
+            // does not enter
             double r = sqrt((0.0 + x * x) + y * y);
             double yb = (r - prm.R) / prm.buffer_scale;
-            double br = 0.5 * prm.buffer_rate * (1.0 + tanh(yb));
-            // initial_conserved_array :182-191 via initial_primitive :87-99
-            double ph = -1.0 * 1.0 / sqrt((x * x + y * y) + prm.s2);
-            double r0 = sqrt(x * x + y * y);
-            double vp = sqrt(-ph) - om * r0;
-            double ui[3] = { 1.0, 1.0 * (vp * (-y / r0)), 1.0 * (vp * (x / r0)) };
+            double br = 0.0;
+            double ui[3] = { 1.0, 0.0, 0.0 };
+            if (yb > -20.0) {
+                br = 0.5 * prm.buffer_rate * (1.0 + tanh(yb));
+                double ph = -1.0 * 1.0 / sqrt((x * x + y * y) + prm.s2);
+                double r0 = sqrt(x * x + y * y);
+                double vp = sqrt(-ph) - om * r0;
+                ui[1] = 1.0 * (vp * (-y / r0));
+                ui[2] = 1.0 * (vp * (x / r0));
+            }
             double sig = pc[0];
             double sg1[3] = { 0.0 / 1.0, sig * ax1, sig * ay1 };
             double sg2[3] = { 0.0 / 1.0, sig * ax2, sig * ay2 };
             double sf1[3] = { 0.0 / 1.0, sig * cenx, sig * ceny };
             double sf2[3] = { 0.0 / 1.0, sig * corx, sig * cory };
+            const double dtdl = dt / dl;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 double ss1 = -u[q] * sr1;
                 double ss2 = -u[q] * sr2;
                 double sb = -(u[q] - ui[q]) * br;
                 double src = sg1[q] + sg2[q] + ss1 + ss2 + sf1[q] + sf2[q] + sb;
-                out[q] = u[q] - (dfx[q] + dfy[q]) * dt / dl + src * dt;
+                out[q] = u[q] - (dfx[q] + dfy[q]) * dtdl + src * dt;
             }
         } else {
             float xf = td.xcf[k - tc.k0], yf = td.ycf[c - tc.c0];
