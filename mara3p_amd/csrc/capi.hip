@@ -324,8 +324,25 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, a.vmax, 1, 0ull);
     hipLaunchKernelGGL(mhd_prim_kernel<R>, dim3(std::min(gc, 4096)), dim3(256), 0, s, a, P);
     hipLaunchKernelGGL(mhd_dt_kernel, dim3(1), dim3(1), 0, s, a);
-    hipLaunchKernelGGL(mhd_faces_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)P, F1, E1, F2, E2, F3, E3);
     FaceE<R> fe{ E1, E2, E3, N, nf };
+    if (!is_fp64<R>::value && N % 8 == 0 && N >= 8) {
+        // column tiles: fluxes and the conserved update in one pass, then the
+        // faces (the FP64 tile needs ~255 registers and runs slower than the
+        // separate passes, so FP64 keeps those)
+        const int TI = N >= 64 ? 32 : N;
+        const int segs = N / TI;
+        if (N % 32 == 0) {
+            const int grid = (N / 8) * (N / 32) * segs;
+            hipLaunchKernelGGL((mhd_tile_kernel<R, 8, 32>), dim3(grid), dim3(256), 0, s, a, (const R*)P, E1, E2, E3, TI);
+        } else {
+            const int grid = (N / 8) * (N / 8) * segs;
+            hipLaunchKernelGGL((mhd_tile_kernel<R, 8, 8>), dim3(grid), dim3(64), 0, s, a, (const R*)P, E1, E2, E3, TI);
+        }
+        if (int st = launch_status("mhd_tile_kernel")) return st;
+        hipLaunchKernelGGL(mhd_face_b_kernel<R>, dim3(gc), dim3(256), 0, s, a, fe);
+        return launch_status("m3p_mhd_advance_ct");
+    }
+    hipLaunchKernelGGL(mhd_faces_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)P, F1, E1, F2, E2, F3, E3);
     hipLaunchKernelGGL(mhd_update_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)F1, (const R*)F2, (const R*)F3, fe);
     return launch_status("m3p_mhd_advance_ct");
 }

@@ -313,6 +313,164 @@ mhd_faces_kernel(MhdArgs a, const R* P, R* F1, R* E1, R* F2, R* E2, R* F3, R* E3
     if (k == N - 1) mhd_one_face<R, 3>(a, P, F3, E3, i, j, N);
 }
 
+// Column-tile form of the flux pass fused with the conserved update.  A
+// workgroup owns TJ x TK columns (j, k) and marches along i over TI planes
+// of one segment.  Each thread keeps a four-plane window of its column's
+// primitives, so the x face above its cell needs one new load per plane;
+// the y and z faces of the plane come from an LDS copy of the plane with a
+// two-cell halo.  Per plane and thread: the x face above the cell, the y
+// and z faces below it (the tile's upper y and z faces by the last row and
+// by the first TJ threads), then U of the cell from the six face fluxes
+// (divergence_difference, src/mesh_cartesian_3d.hpp:290-301).  Face fluxes
+// never leave the workgroup; the transverse face electric fields go to the
+// E arrays for the face update.  Every face is computed from the same
+// operands in the same order as mhd_one_face, so a face shared by two
+// tiles or segments has one value.
+template<typename R, int TJ, int TK>
+__global__ void __launch_bounds__(TJ * TK)
+mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
+{
+    constexpr int NT = TJ * TK;
+    constexpr int EJ = TJ + 4, EK = TK + 4;
+    const int N = a.N;
+    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
+    const int tiles_k = N / TK, tiles_j = N / TJ;
+    int w = blockIdx.x;
+    const int k0 = (w % tiles_k) * TK; w /= tiles_k;
+    const int j0 = (w % tiles_j) * TJ; w /= tiles_j;
+    const int i0 = w * TI;
+    const int t = threadIdx.x, jl = t / TK, kl = t % TK;
+    const int j = j0 + jl, k = k0 + kl;
+
+    __shared__ R Pl[8][EJ][EK];       // primitives of plane i, (j0 - 2 .. j0 + TJ + 1) x (k0 - 2 .. k0 + TK + 1)
+    __shared__ R F2s[5][TJ + 1][TK];  // y-face fluxes of plane i, faces j0 .. j0 + TJ
+    __shared__ R F3s[5][TJ][TK + 1];  // z-face fluxes of plane i, faces k0 .. k0 + TK
+
+    const double dt = *a.dt;
+    const R theta = R(a.theta), gamma = R(a.gamma);
+    auto cell = [&] (int ii, int jj, int kk) {
+        return ((long)wrapi(ii, N) * N + wrapi(jj, N)) * N + wrapi(kk, N);
+    };
+    auto loadP = [&] (long c, R* p) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) p[q] = P[q * N3 + c];
+    };
+    // face states from four cells along the face normal (lower two, upper two)
+    auto states = [&] (const R* c0, const R* c1, const R* c2, const R* c3, R* pl, R* pr) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            pl[q] = c1[q] + R(0.5) * plm(c0[q], c1[q], c2[q], theta);
+            pr[q] = c2[q] - R(0.5) * plm(c1[q], c2[q], c3[q], theta);
+        }
+    };
+
+    // x window: Pm = plane i-1, P0 = i, Pp = i+1; warm-up face at i0
+    R Pm[8], P0[8], Pp[8], Fx_lo[5];
+    {
+        R Pm2[8], pl[8], pr[8], e[3];
+        loadP(cell(i0 - 2, j, k), Pm2);
+        loadP(cell(i0 - 1, j, k), Pm);
+        loadP(cell(i0, j, k), P0);
+        loadP(cell(i0 + 1, j, k), Pp);
+        states(Pm2, Pm, P0, Pp, pl, pr);
+        const long f = ((long)i0 * N + j) * N + k;
+        Mhd<R>::template riemann<1>(pl, pr, R(a.b1[f]), gamma, Fx_lo, e);
+        if (i0 == 0) { E1[f] = e[1]; E1[nf + f] = e[2]; }
+    }
+
+    for (int il = 0; il < TI; ++il) {
+        const int i = i0 + il;
+        R Pn[8];
+        loadP(cell(i + 2, j, k), Pn);
+
+        // LDS plane i: interior from the window, halo ring from memory
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Pl[q][jl + 2][kl + 2] = P0[q];
+        for (int r = t; r < 4 * EK + 4 * TJ; r += NT) {
+            int rj, rk;
+            if (r < 2 * EK)      { rj = r / EK; rk = r % EK; }
+            else if (r < 4 * EK) { rj = TJ + 2 + (r - 2 * EK) / EK; rk = (r - 2 * EK) % EK; }
+            else {
+                const int s2 = r - 4 * EK;
+                rj = 2 + s2 / 4;
+                const int side = s2 % 4;
+                rk = side < 2 ? side : TK + side;
+            }
+            const long c = cell(i, j0 - 2 + rj, k0 - 2 + rk);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Pl[q][rj][rk] = P[q * N3 + c];
+        }
+
+        // x face above the cell (face i + 1)
+        R Fx_hi[5];
+        {
+            R pl[8], pr[8], e[3];
+            states(Pm, P0, Pp, Pn, pl, pr);
+            const long f = ((long)(i + 1) * N + j) * N + k;
+            Mhd<R>::template riemann<1>(pl, pr, R(a.b1[f]), gamma, Fx_hi, e);
+            E1[f] = e[1];
+            E1[nf + f] = e[2];
+        }
+        __syncthreads();
+
+        // y face below the cell (face j), and the tile's upper y face by the last row
+        auto yface = [&] (int jf /* local face row 0..TJ */) {
+            R c0[8], c1[8], c2[8], c3[8], pl[8], pr[8], f[5], e[3];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                c0[q] = Pl[q][jf][kl + 2]; c1[q] = Pl[q][jf + 1][kl + 2];
+                c2[q] = Pl[q][jf + 2][kl + 2]; c3[q] = Pl[q][jf + 3][kl + 2];
+            }
+            states(c0, c1, c2, c3, pl, pr);
+            const long fi = ((long)i * (N + 1) + (j0 + jf)) * N + k;
+            Mhd<R>::template riemann<2>(pl, pr, R(a.b2[fi]), gamma, f, e);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) F2s[q][jf][kl] = f[q];
+            if (jf < TJ || j0 + jf == N) { E2[fi] = e[0]; E2[nf + fi] = e[2]; }
+        };
+        yface(jl);
+        if (jl == TJ - 1) yface(TJ);
+
+        // z face below the cell (face k), and the tile's upper z faces by threads 0 .. TJ-1
+        auto zface = [&] (int row, int kf /* local face column 0..TK */) {
+            R c0[8], c1[8], c2[8], c3[8], pl[8], pr[8], f[5], e[3];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                c0[q] = Pl[q][row + 2][kf]; c1[q] = Pl[q][row + 2][kf + 1];
+                c2[q] = Pl[q][row + 2][kf + 2]; c3[q] = Pl[q][row + 2][kf + 3];
+            }
+            states(c0, c1, c2, c3, pl, pr);
+            const long fi = ((long)i * N + (j0 + row)) * (N + 1) + (k0 + kf);
+            Mhd<R>::template riemann<3>(pl, pr, R(a.b3[fi]), gamma, f, e);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) F3s[q][row][kf] = f[q];
+            if (kf < TK || k0 + kf == N) { E3[fi] = e[0]; E3[nf + fi] = e[1]; }
+        };
+        zface(jl, kl);
+        if (t < TJ) zface(t, TK);
+        __syncthreads();
+
+        // conserved update of cell (i, j, k)
+        {
+            const long c = ((long)i * N + j) * N + k;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                double d1 = double(Fx_hi[q]) - double(Fx_lo[q]);
+                double d2 = double(F2s[q][jl + 1][kl]) - double(F2s[q][jl][kl]);
+                double d3 = double(F3s[q][jl][kl + 1]) - double(F3s[q][jl][kl]);
+                double r = a.uc[q * N3 + c] - (d1 + d2 + d3) * dt / a.dl;
+                if (a.u0) r = a.u0[q * N3 + c] * a.rk_b + r * (1.0 - a.rk_b);
+                a.uo[q * N3 + c] = r;
+            }
+        }
+        __syncthreads();   // the plane buffers are rewritten by the next plane
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { Pm[q] = P0[q]; P0[q] = Pp[q]; Pp[q] = Pn[q]; }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) Fx_lo[q] = Fx_hi[q];
+    }
+}
+
 // Face-centred transverse electric field of axis-A faces, by component.
 template<typename R>
 struct FaceE {
@@ -409,6 +567,26 @@ mhd_update_kernel(MhdArgs a, const R* F1, const R* F2, const R* F3, FaceE<R> fe)
         if (a.u0) r = a.u0[q * N3 + c] * a.rk_b + r * (1.0 - a.rk_b);
         a.uo[q * N3 + c] = r;
     }
+    mhd_update_face<R, 1>(a, fe, dt, i, j, k);
+    mhd_update_face<R, 2>(a, fe, dt, i, j, k);
+    mhd_update_face<R, 3>(a, fe, dt, i, j, k);
+    if (i == N - 1) mhd_update_face<R, 1>(a, fe, dt, N, j, k);
+    if (j == N - 1) mhd_update_face<R, 2>(a, fe, dt, i, N, k);
+    if (k == N - 1) mhd_update_face<R, 3>(a, fe, dt, i, j, N);
+}
+
+// The face part of mhd_update_kernel alone (the conserved update is done by
+// mhd_tile_kernel).
+template<typename R>
+__global__ void __launch_bounds__(256)
+mhd_face_b_kernel(MhdArgs a, FaceE<R> fe)
+{
+    const int N = a.N;
+    const long N3 = (long)N * N * N;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N3) return;
+    const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);
+    const double dt = *a.dt;
     mhd_update_face<R, 1>(a, fe, dt, i, j, k);
     mhd_update_face<R, 2>(a, fe, dt, i, j, k);
     mhd_update_face<R, 3>(a, fe, dt, i, j, k);

@@ -81,7 +81,7 @@ struct MinidiskParams {
 template<typename R>
 struct MinidiskScheme {
     static constexpr int NC = 3;
-    static constexpr int kWavesPerEU = is_fp64<R>::value ? 3 : 4;
+    static constexpr int kWavesPerEU = is_fp64<R>::value ? 2 : 4;
     using Real = R;
     using Params = MinidiskParams;
     // Vertex coordinates of the tile's rows/columns in FP64 (reference formula),
