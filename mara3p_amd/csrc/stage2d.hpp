@@ -481,8 +481,14 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
 #pragma unroll
                     for (int m = 0; m < NC; ++m) U0[m] = z0[(size_t)m * n * n];
                 }
+                if constexpr (is_fp64<R>::value) {
 #pragma unroll
-                for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = U0[m] * a.rk_b + res[m] * (1.0 - a.rk_b);
+                    for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = U0[m] * a.rk_b + res[m] * (1.0 - a.rk_b);
+                } else {
+                    const double wb = 1.0 - a.rk_b;
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = fma(U0[m], a.rk_b, res[m] * wb);
+                }
             } else {
 #pragma unroll
                 for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = res[m];
