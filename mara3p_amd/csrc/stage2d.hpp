@@ -290,7 +290,12 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
     const int n = a.n;
     const int tiles_i = n / H, tiles_j = n / TJ;
     const int per_block = tiles_i * tiles_j;
-    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    // RK stages with a base state (the second stage) visit the tiles in the
+    // reverse order of the first: the tiles a stage reads first are the ones
+    // the previous stage wrote last, which are the likeliest to still be in
+    // the memory-side cache.
+    const int w0 = xcd_remap(blockIdx.x, gridDim.x);
+    const int w = a.u0 ? (int)gridDim.x - 1 - w0 : w0;
     TileCtx tc;
     tc.b = w / per_block;
     const int rem = w % per_block;
