@@ -338,7 +338,9 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
     int w = blockIdx.x;
     const int k0 = (w % tiles_k) * TK; w /= tiles_k;
     const int j0 = (w % tiles_j) * TJ; w /= tiles_j;
-    const int i0 = w * TI;
+    // segments from the top plane down: the primitive pass wrote the high
+    // planes last, so they are the likeliest to still be cached
+    const int i0 = (N / TI - 1 - w) * TI;
     const int t = threadIdx.x, jl = t / TK, kl = t % TK;
     const int j = j0 + jl, k = k0 + kl;
 
@@ -583,6 +585,7 @@ mhd_face_b_kernel(MhdArgs a, FaceE<R> fe)
 {
     const int N = a.N;
     const long N3 = (long)N * N * N;
+    // cells from the bottom up: the tile pass wrote the low planes last
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N3) return;
     const int i = (int)(c / ((long)N * N)), j = (int)(c / N % N), k = (int)(c % N);

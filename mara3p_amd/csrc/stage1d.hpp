@@ -47,7 +47,12 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
 
     // The lane's own zone of the next item is fetched while the current item
     // is computed (software pipelining across the work list).
+    // The second RK stage (u0 given) walks the list backwards: its first
+    // reads are what the first stage wrote last (likeliest to be cached).
+    const bool rev = a.u0 != nullptr;
+    auto slot = [&] (long it) { return rev ? items - 1 - it : it; };
     auto fetch = [&] (long it, double* raw) {
+        it = slot(it);
         const double* src = a.u + (size_t)(it / tiles) * NC * n + (it % tiles) * TJ + t;
 #pragma unroll
         for (int q = 0; q < NC; ++q) raw[q] = src[(size_t)q * n];
@@ -57,8 +62,9 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
 
     bool ok = true;
     for (long item = blockIdx.x; item < items; item += gridDim.x) {
-        const int b = (int)(item / tiles);
-        const int c0 = (int)(item % tiles) * TJ;
+        const long it = slot(item);
+        const int b = (int)(it / tiles);
+        const int c0 = (int)(it % tiles) * TJ;
         const double dl = a.dl_blk ? a.dl_blk[b] : a.dl;
         double uz[NC], U0[NC];
 #pragma unroll
