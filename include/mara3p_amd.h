@@ -157,6 +157,60 @@ int  m3p_mhd_advance_ct(const double* uc, const double* bf1, const double* bf2, 
  * src/mesh_cartesian_3d.hpp:290-301) into out (device double). */
 int  m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, int N, double* out, void* stream);
 
+/* mhd_scheme_v2 array functions (src/scheme_mhd_v2.hpp:71-210, bodies
+ * src/scheme_mhd_v2.cpp:75-278) over one block, FP64 in the reference's
+ * arithmetic, adiabatic index 5/3 (src/scheme_mhd_v2.cpp:48).  Device arrays,
+ * row-major, components outermost: cell tuples [nq][s1][s2][s3] (5 conserved,
+ * 8 primitive (rho, v1, v2, v3, p, B1, B2, B3), 8 Godunov data = 5 fluxes +
+ * 3 face electric field components); face and edge arrays are scalars in the
+ * reference's shapes: for cells (n1, n2, n3), x faces (n1+1, n2, n3), x edges
+ * (n1, n2+1, n3+1), and the other axes by symmetry.  Asynchronous on stream,
+ * except m3p_mhd_v2_primitive_array, which waits and returns the status. */
+
+/* primitive_array :75-81 (face_to_cell, src/mesh_cartesian_3d.hpp:235-245):
+ * uc [5] (n1, n2, n3) + faces -> pc [8] (n1, n2, n3).  Blocking;
+ * M3P_ERR_NEGATIVE_DENSITY / _PRESSURE where mhd::recover_primitive throws
+ * (src/physics_mhd.hpp:296-300), for one offending cell. */
+int m3p_mhd_v2_primitive_array(const double* uc, const double* bf1, const double* bf2, const double* bf3,
+                               int n1, int n2, int n3, double* pc, void* stream);
+/* godunov_fluxes :168-185 (first-order face states, HLLE): pc [8] (m1, m2, m3)
+ * extended by one on every axis; bf1 (m1-1, m2, m3), bf2 (m1, m2-1, m3),
+ * bf3 (m1, m2, m3-1) -> gf1, gf2, gf3 [8] of the same face shapes. */
+int m3p_mhd_v2_godunov_fluxes(const double* pc, const double* bf1, const double* bf2, const double* bf3,
+                              int m1, int m2, int m3, double* gf1, double* gf2, double* gf3, void* stream);
+/* electromotive_forces :191-214: gf of cells (m1, m2, m3) -> edge fields
+ * ee1 (m1, m2-1, m3-1), ee2 (m1-1, m2, m3-1), ee3 (m1-1, m2-1, m3). */
+int m3p_mhd_v2_electromotive_forces(const double* gf1, const double* gf2, const double* gf3, int m1, int m2, int m3,
+                                    double* ee1, double* ee2, double* ee3, void* stream);
+/* updated_conserved_density :220-241: uc [5] (n1, n2, n3); gf extended by one
+ * on the transverse axes (gf1 (n1+1, n2+2, n3+2), ...) -> out [5] (n1, n2, n3). */
+int m3p_mhd_v2_updated_conserved_density(const double* uc, const double* gf1, const double* gf2, const double* gf3,
+                                         int n1, int n2, int n3, double dt, double dl, double* out, void* stream);
+/* curl :247-258: edges e1 (n1, n2+1, n3+1), e2, e3 -> faces over a unit time. */
+int m3p_mhd_v2_curl(const double* ee1, const double* ee2, const double* ee3, int n1, int n2, int n3, double dl,
+                    double* bf1, double* bf2, double* bf3, void* stream);
+/* updated_magnetic_flux_density :264-278: faces of cells (n1, n2, n3); edges as
+ * electromotive_forces returns them for gf extended by one (ee1 (n1+2, n2+1,
+ * n3+1), ...; the longitudinal end layers are not read) -> bo1, bo2, bo3. */
+int m3p_mhd_v2_updated_magnetic_flux_density(const double* bf1, const double* bf2, const double* bf3,
+                                             const double* ee1, const double* ee2, const double* ee3,
+                                             int n1, int n2, int n3, double dt, double dl,
+                                             double* bo1, double* bo2, double* bo3, void* stream);
+/* extend_cell_primitive_variables :126-135: pc_blocks is a host array of 27
+ * device pointers, the blocks [8] (bs, bs, bs) of a 3 x 3 x 3 neighbourhood in
+ * row-major order of the offsets (mesh::neighbors_27,
+ * src/mesh_cartesian_3d.hpp:355-364) -> out [8] (bs+2c, bs+2c, bs+2c),
+ * 0 <= count <= bs. */
+int m3p_mhd_v2_extend_cell_primitive_variables(const double* const* pc_blocks, int block_size, int count,
+                                               double* out, void* stream);
+/* extend_face_magnetic_flux_density :141-162: bf_blocks is a host array of 27
+ * device pointers; entries 9a .. 9a+8 are the axis-a face arrays of the 3 x 3
+ * square of blocks normal to axis a (mesh::neighbors_9, :384-395, row-major)
+ * -- the component the reference takes from each of its 27 face triplets ->
+ * bf1 (bs+1, bs+2c, bs+2c), bf2 (bs+2c, bs+1, bs+2c), bf3 (bs+2c, bs+2c, bs+1). */
+int m3p_mhd_v2_extend_face_magnetic_flux_density(const double* const* bf_blocks, int block_size, int count,
+                                                 double* bf1, double* bf2, double* bf3, void* stream);
+
 /* Special-relativistic hydrodynamics (pointwise, FP64) -------------------- */
 /* srhd::recover_primitive, src/physics_srhd.hpp:224-279 (Newton on the
  * pressure, <= 50 iterations, tolerance 1e-12): u, p [5][nzones]; blocking;

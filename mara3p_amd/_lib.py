@@ -39,6 +39,16 @@ class NegativeDensityError(M3PError, ValueError):
     ("euler::recover_primitive (negative density)", src/physics_euler.hpp:158-161)."""
 
 
+class NegativePressureError(M3PError, ValueError):
+    """Raised where the reference throws std::invalid_argument for a negative
+    pressure (mhd::recover_primitive, src/physics_mhd.hpp:296-300)."""
+
+
+class NoConvergenceError(M3PError, ValueError):
+    """Raised where the reference throws std::invalid_argument for a root finder
+    that does not converge (srhd::recover_primitive, src/physics_srhd.hpp:273-277)."""
+
+
 class MinidiskData(ctypes.Structure):
     """problems/minidisk.hpp:80-96 solver_data_t (defaults :99-122)."""
     _fields_ = [
@@ -103,6 +113,14 @@ SIGNATURES = {
     "m3p_mhd_advance_ct": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P,
                                 _P, _L, _I, _P]),
     "m3p_mhd_max_div_b": (_I, [_P, _P, _P, _I, _P, _P]),
+    "m3p_mhd_v2_primitive_array": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P]),
+    "m3p_mhd_v2_godunov_fluxes": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
+    "m3p_mhd_v2_electromotive_forces": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
+    "m3p_mhd_v2_updated_conserved_density": (_I, [_P, _P, _P, _P, _I, _I, _I, _D, _D, _P, _P]),
+    "m3p_mhd_v2_curl": (_I, [_P, _P, _P, _I, _I, _I, _D, _P, _P, _P, _P]),
+    "m3p_mhd_v2_updated_magnetic_flux_density": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _P, _P, _P, _P]),
+    "m3p_mhd_v2_extend_cell_primitive_variables": (_I, [_P, _I, _I, _P, _P]),
+    "m3p_mhd_v2_extend_face_magnetic_flux_density": (_I, [_P, _I, _I, _P, _P, _P, _P]),
     "m3p_srhd_recover_primitive_array": (_I, [_P, _P, _L, _D, _D, _P]),
     "m3p_euler1d_stage_blocks": (_I, [_P, _P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _I, _P]),
     "m3p_amr_euler2d_stage": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _I, _I, _P]),
@@ -152,6 +170,10 @@ def check(status):
     msg = last_error()
     if status == M3P_ERR_NEGATIVE_DENSITY:
         raise NegativeDensityError(status, msg)
+    if status == M3P_ERR_NEGATIVE_PRESSURE:
+        raise NegativePressureError(status, msg)
+    if status == M3P_ERR_NO_CONVERGENCE:
+        raise NoConvergenceError(status, msg)
     raise M3PError(status, msg)
 
 

@@ -1,5 +1,6 @@
 // C ABI of the MI355X Godunov block update (declarations: include/mara3p_amd.h).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -10,6 +11,7 @@
 #include "viscous2d.hpp"
 #include "elementwise.hpp"
 #include "mhd.hpp"
+#include "mhd_v2.hpp"
 #include "srhd.hpp"
 #include "stage1d.hpp"
 #include "stage2d.hpp"
@@ -838,6 +840,132 @@ int m3p_locally_isothermal_updated_conserved(const double* uc, const double* pe,
         st = launch_stage2d<IsoTableScheme<float>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<IsoTableScheme<float>>{uc, pe, n}, a, s);
     if (st) return st;
     return finish_blocking(stream);
+}
+
+// ---------------------------------------------------------------------------
+// mhd_scheme_v2 array functions
+static dim3 v2_grid(long n, int axes)
+{
+    return dim3((unsigned)std::max<long>(1, std::min<long>((n + 255) / 256, 16384)), (unsigned)axes);
+}
+
+static bool v2_shape_ok(int a, int b, int c, int least) { return a >= least && b >= least && c >= least; }
+
+int m3p_mhd_v2_primitive_array(const double* uc, const double* bf1, const double* bf2, const double* bf3,
+                               int n1, int n2, int n3, double* pc, void* stream)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!uc || !bf1 || !bf2 || !bf3 || !pc) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_v2_primitive_array: NULL array");
+    if (!v2_shape_ok(n1, n2, n3, 1)) return fail(M3P_ERR_BAD_SHAPE, "m3p_mhd_v2_primitive_array: empty block");
+    hipLaunchKernelGGL(mhd_v2_prim_kernel, v2_grid((long)n1 * n2 * n3, 1), dim3(256), 0, (hipStream_t)stream,
+                       n1, n2, n3, uc, bf1, bf2, bf3, pc, ds->err);
+    if (int st = launch_status("mhd_v2_prim_kernel")) return st;
+    int code = 0, block = 0, zone = 0;
+    if (int st = m3p_check_errors(stream, &code, &block, &zone)) return st;
+    if (code == M3P_ERR_NEGATIVE_DENSITY) return fail(code, "mhd::recover_primitive (negative density)");
+    if (code == M3P_ERR_NEGATIVE_PRESSURE) return fail(code, "mhd::recover_primitive (negative pressure)");
+    return code ? fail(code, "m3p: kernel error " + std::to_string(code)) : M3P_OK;
+}
+
+int m3p_mhd_v2_godunov_fluxes(const double* pc, const double* bf1, const double* bf2, const double* bf3,
+                              int m1, int m2, int m3, double* gf1, double* gf2, double* gf3, void* stream)
+{
+    if (!pc || !bf1 || !bf2 || !bf3 || !gf1 || !gf2 || !gf3)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_v2_godunov_fluxes: NULL array");
+    if (!v2_shape_ok(m1, m2, m3, 2)) return fail(M3P_ERR_BAD_SHAPE, "m3p_mhd_v2_godunov_fluxes: cells must be >= 2 per axis");
+    const long nmax = (long)m1 * m2 * m3;
+    hipLaunchKernelGGL(mhd_v2_godunov_kernel, v2_grid(nmax, 3), dim3(256), 0, (hipStream_t)stream,
+                       m1, m2, m3, pc, bf1, bf2, bf3, gf1, gf2, gf3);
+    return launch_status("mhd_v2_godunov_kernel");
+}
+
+int m3p_mhd_v2_electromotive_forces(const double* gf1, const double* gf2, const double* gf3, int m1, int m2, int m3,
+                                    double* ee1, double* ee2, double* ee3, void* stream)
+{
+    if (!gf1 || !gf2 || !gf3 || !ee1 || !ee2 || !ee3)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_v2_electromotive_forces: NULL array");
+    if (!v2_shape_ok(m1, m2, m3, 2)) return fail(M3P_ERR_BAD_SHAPE, "m3p_mhd_v2_electromotive_forces: cells must be >= 2 per axis");
+    hipLaunchKernelGGL(mhd_v2_emf_kernel, v2_grid((long)m1 * m2 * m3, 3), dim3(256), 0, (hipStream_t)stream,
+                       m1, m2, m3, gf1, gf2, gf3, ee1, ee2, ee3);
+    return launch_status("mhd_v2_emf_kernel");
+}
+
+int m3p_mhd_v2_updated_conserved_density(const double* uc, const double* gf1, const double* gf2, const double* gf3,
+                                         int n1, int n2, int n3, double dt, double dl, double* out, void* stream)
+{
+    if (!uc || !gf1 || !gf2 || !gf3 || !out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_v2_updated_conserved_density: NULL array");
+    if (out == uc) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_v2_updated_conserved_density: out must not alias uc");
+    if (!v2_shape_ok(n1, n2, n3, 1)) return fail(M3P_ERR_BAD_SHAPE, "m3p_mhd_v2_updated_conserved_density: empty block");
+    hipLaunchKernelGGL(mhd_v2_update_cells_kernel, v2_grid((long)n1 * n2 * n3, 1), dim3(256), 0, (hipStream_t)stream,
+                       n1, n2, n3, uc, gf1, gf2, gf3, dt, dl, out);
+    return launch_status("mhd_v2_update_cells_kernel");
+}
+
+int m3p_mhd_v2_curl(const double* ee1, const double* ee2, const double* ee3, int n1, int n2, int n3, double dl,
+                    double* bf1, double* bf2, double* bf3, void* stream)
+{
+    if (!ee1 || !ee2 || !ee3 || !bf1 || !bf2 || !bf3) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_v2_curl: NULL array");
+    if (!v2_shape_ok(n1, n2, n3, 1)) return fail(M3P_ERR_BAD_SHAPE, "m3p_mhd_v2_curl: empty block");
+    hipLaunchKernelGGL(mhd_v2_faces_kernel, v2_grid((long)(n1 + 1) * (n2 + 1) * (n3 + 1), 3), dim3(256), 0,
+                       (hipStream_t)stream, n1, n2, n3, 0, ee1, ee2, ee3, (const double*)nullptr,
+                       (const double*)nullptr, (const double*)nullptr, 1.0, dl, bf1, bf2, bf3);
+    return launch_status("mhd_v2_faces_kernel");
+}
+
+int m3p_mhd_v2_updated_magnetic_flux_density(const double* bf1, const double* bf2, const double* bf3,
+                                             const double* ee1, const double* ee2, const double* ee3,
+                                             int n1, int n2, int n3, double dt, double dl,
+                                             double* bo1, double* bo2, double* bo3, void* stream)
+{
+    if (!bf1 || !bf2 || !bf3 || !ee1 || !ee2 || !ee3 || !bo1 || !bo2 || !bo3)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_v2_updated_magnetic_flux_density: NULL array");
+    if (bo1 == bf1 || bo2 == bf2 || bo3 == bf3)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_v2_updated_magnetic_flux_density: outputs must not alias inputs");
+    if (!v2_shape_ok(n1, n2, n3, 1)) return fail(M3P_ERR_BAD_SHAPE, "m3p_mhd_v2_updated_magnetic_flux_density: empty block");
+    hipLaunchKernelGGL(mhd_v2_faces_kernel, v2_grid((long)(n1 + 1) * (n2 + 1) * (n3 + 1), 3), dim3(256), 0,
+                       (hipStream_t)stream, n1, n2, n3, 1, ee1, ee2, ee3, bf1, bf2, bf3, dt, dl, bo1, bo2, bo3);
+    return launch_status("mhd_v2_faces_kernel");
+}
+
+static int v2_blocks(const double* const* src, Ptr27& dst, const char* what)
+{
+    if (!src) return fail(M3P_ERR_BAD_ARGUMENT, std::string(what) + ": NULL block list");
+    for (int m = 0; m < 27; ++m) {
+        if (!src[m]) return fail(M3P_ERR_BAD_ARGUMENT, std::string(what) + ": NULL block pointer");
+        dst.p[m] = src[m];
+    }
+    return M3P_OK;
+}
+
+int m3p_mhd_v2_extend_cell_primitive_variables(const double* const* pc_blocks, int block_size, int count,
+                                               double* out, void* stream)
+{
+    const char* what = "m3p_mhd_v2_extend_cell_primitive_variables";
+    Ptr27 blk;
+    if (int st = v2_blocks(pc_blocks, blk, what)) return st;
+    if (!out) return fail(M3P_ERR_BAD_ARGUMENT, std::string(what) + ": NULL output");
+    if (block_size < 1 || count < 0 || count > block_size)
+        return fail(M3P_ERR_BAD_SHAPE, std::string(what) + ": need 0 <= count <= block_size");
+    const long m = block_size + 2L * count;
+    hipLaunchKernelGGL(mhd_v2_extend_cells_kernel, v2_grid(m * m * m * 8, 1), dim3(256), 0, (hipStream_t)stream,
+                       blk, block_size, count, 8, out);
+    return launch_status("mhd_v2_extend_cells_kernel");
+}
+
+int m3p_mhd_v2_extend_face_magnetic_flux_density(const double* const* bf_blocks, int block_size, int count,
+                                                 double* bf1, double* bf2, double* bf3, void* stream)
+{
+    const char* what = "m3p_mhd_v2_extend_face_magnetic_flux_density";
+    Ptr27 blk;
+    if (int st = v2_blocks(bf_blocks, blk, what)) return st;
+    if (!bf1 || !bf2 || !bf3) return fail(M3P_ERR_BAD_ARGUMENT, std::string(what) + ": NULL output");
+    if (block_size < 1 || count < 0 || count > block_size)
+        return fail(M3P_ERR_BAD_SHAPE, std::string(what) + ": need 0 <= count <= block_size");
+    const long m = block_size + 2L * count;
+    hipLaunchKernelGGL(mhd_v2_extend_faces_kernel, v2_grid((block_size + 1L) * m * m, 3), dim3(256), 0,
+                       (hipStream_t)stream, blk, block_size, count, bf1, bf2, bf3);
+    return launch_status("mhd_v2_extend_faces_kernel");
 }
 
 } // extern "C"

@@ -66,6 +66,14 @@ _SIG = {
     "orc_mhd_riemann_hlle": (None, [_P, _P, _D, _I, _D, _P, _P]),
     "orc_mhd_advance_ct": (_I, [_I, _P, _P, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P, _P]),
     "orc_mhd_rk2": (_I, [_I, _P, _P, _P, _P, _D, _D, _D, _D, _P]),
+    "orc_mhd_v2_primitive_array": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
+    "orc_mhd_v2_godunov_fluxes": (None, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "orc_mhd_v2_electromotive_forces": (None, [_I, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "orc_mhd_v2_updated_conserved_density": (None, [_I, _I, _I, _P, _P, _P, _P, _D, _D, _P]),
+    "orc_mhd_v2_curl": (None, [_I, _I, _I, _P, _P, _P, _D, _P, _P, _P]),
+    "orc_mhd_v2_updated_magnetic_flux_density": (None, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P]),
+    "orc_mhd_v2_extend_cells": (None, [_P, _I, _I, _I, _P]),
+    "orc_mhd_v2_extend_faces": (None, [_P, _I, _I, _P, _P, _P]),
     "orc_srhd_recover_primitive": (_I, [_P, _D, _D, _P]),
     "orc_srhd_conserved_density": (None, [_P, _D, _P]),
     "orc_srhd_flux": (None, [_P, _I, _D, _P]),

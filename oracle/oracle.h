@@ -109,6 +109,23 @@ int orc_mhd_advance_ct(int N, const double* uc, const double* bf1, const double*
                        double* uo, double* bo1, double* bo2, double* bo3, double* dt_out);
 int orc_mhd_rk2(int N, double* uc, double* bf1, double* bf2, double* bf3, double dl, double cfl, double theta,
                 double gamma, double* dts);
+/* mhd_scheme_v2 array functions (src/scheme_mhd_v2.cpp:75-278), adiabatic index 5/3 */
+int  orc_mhd_v2_primitive_array(int n1, int n2, int n3, const double* uc, const double* bf1, const double* bf2,
+                                const double* bf3, double* pc);
+void orc_mhd_v2_godunov_fluxes(int m1, int m2, int m3, const double* pc, const double* bf1, const double* bf2,
+                               const double* bf3, double* gf1, double* gf2, double* gf3);
+void orc_mhd_v2_electromotive_forces(int m1, int m2, int m3, const double* gf1, const double* gf2, const double* gf3,
+                                     double* ee1, double* ee2, double* ee3);
+void orc_mhd_v2_updated_conserved_density(int n1, int n2, int n3, const double* uc, const double* gf1,
+                                          const double* gf2, const double* gf3, double dt, double dl, double* out);
+void orc_mhd_v2_curl(int n1, int n2, int n3, const double* e1, const double* e2, const double* e3, double dl,
+                     double* b1, double* b2, double* b3);
+void orc_mhd_v2_updated_magnetic_flux_density(int n1, int n2, int n3, const double* bf1, const double* bf2,
+                                              const double* bf3, const double* ee1, const double* ee2,
+                                              const double* ee3, double dt, double dl, double* b1, double* b2,
+                                              double* b3);
+void orc_mhd_v2_extend_cells(const double* const* blocks, int bs, int count, int nq, double* out);
+void orc_mhd_v2_extend_faces(const double* const* blocks, int bs, int count, double* bf1, double* bf2, double* bf3);
 
 /* ---------------- special-relativistic hydrodynamics (oracle_srhd.c) ---------------- */
 /* primitive (rho, u1, u2, u3, p), u = gamma beta; conserved (D, S1, S2, S3, tau) */

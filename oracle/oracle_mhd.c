@@ -334,3 +334,211 @@ int orc_mhd_rk2(int N, double* uc, double* bf1, double* bf2, double* bf3, double
     for (int a = 0; a < 6; ++a) free(b[a]);
     return st;
 }
+
+/* ------------------------------------------------------------------------ */
+/* mhd_scheme_v2 array functions (src/scheme_mhd_v2.cpp:75-278) on blocks of
+ * any shape, adiabatic index 5/3 (:48).  Logical layout, row-major: cell
+ * tuples last ([..][8] primitives, [..][5] conserved, [..][8] Godunov data =
+ * the 5 flux components then the 3 components of the face electric field);
+ * face and edge arrays are scalars.  For cells (n1, n2, n3): x faces
+ * (n1+1, n2, n3), x edges (n1, n2+1, n3+1), and so on by symmetry. */
+
+static const double V2_GAMMA = 5. / 3;
+
+#define IX3(i, j, k, s2, s3) (((long)(i) * (s2) + (j)) * (s3) + (k))
+
+/* primitive_array :75-81 with face_to_cell (src/mesh_cartesian_3d.hpp:235-245).
+ * Returns the status of the first cell in row-major order that fails. */
+int orc_mhd_v2_primitive_array(int n1, int n2, int n3, const double* uc, const double* bf1, const double* bf2,
+                               const double* bf3, double* pc)
+{
+    int status = 0;
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j)
+            for (int k = 0; k < n3; ++k) {
+                long c = IX3(i, j, k, n2, n3);
+                double b[3] = { 0.5 * (bf1[IX3(i, j, k, n2, n3)] + bf1[IX3(i + 1, j, k, n2, n3)]),
+                                0.5 * (bf2[IX3(i, j, k, n2 + 1, n3)] + bf2[IX3(i, j + 1, k, n2 + 1, n3)]),
+                                0.5 * (bf3[IX3(i, j, k, n2, n3 + 1)] + bf3[IX3(i, j, k + 1, n2, n3 + 1)]) };
+                int st = orc_mhd_recover_primitive(&uc[c * 5], b, V2_GAMMA, &pc[c * 8]);
+                if (st && !status) status = st;
+            }
+    return status;
+}
+
+/* godunov_fluxes :168-185: first-order face states (the two cells either side
+ * of the face), HLLE with the face's own longitudinal field.  pc (m1, m2, m3);
+ * bf1 / gf1 (m1-1, m2, m3), bf2 / gf2 (m1, m2-1, m3), bf3 / gf3 (m1, m2, m3-1). */
+void orc_mhd_v2_godunov_fluxes(int m1, int m2, int m3, const double* pc, const double* bf1, const double* bf2,
+                               const double* bf3, double* gf1, double* gf2, double* gf3)
+{
+    for (int a = 0; a < 3; ++a) {
+        const int s1 = m1 - (a == 0), s2 = m2 - (a == 1), s3 = m3 - (a == 2);
+        const double* bf = a == 0 ? bf1 : (a == 1 ? bf2 : bf3);
+        double* gf = a == 0 ? gf1 : (a == 1 ? gf2 : gf3);
+        for (int i = 0; i < s1; ++i)
+            for (int j = 0; j < s2; ++j)
+                for (int k = 0; k < s3; ++k) {
+                    long f = IX3(i, j, k, s2, s3);
+                    const double* pl = &pc[IX3(i, j, k, m2, m3) * 8];
+                    const double* pr = &pc[IX3(i + (a == 0), j + (a == 1), k + (a == 2), m2, m3) * 8];
+                    orc_mhd_riemann_hlle(pl, pr, bf[f], a + 1, V2_GAMMA, &gf[f * 8], &gf[f * 8 + 5]);
+                }
+    }
+}
+
+/* electromotive_forces :191-214: each edge field is the mean of two face
+ * estimates, each averaged along its other transverse axis (nd::adjacent_mean,
+ * src/core_ndarray_ops.hpp:84-90).  gf from a cell array (m1, m2, m3);
+ * ee1 (m1, m2-1, m3-1), ee2 (m1-1, m2, m3-1), ee3 (m1-1, m2-1, m3). */
+void orc_mhd_v2_electromotive_forces(int m1, int m2, int m3, const double* gf1, const double* gf2, const double* gf3,
+                                     double* ee1, double* ee2, double* ee3)
+{
+#define G1(c, i, j, k) gf1[IX3(i, j, k, m2, m3) * 8 + (c)]
+#define G2(c, i, j, k) gf2[IX3(i, j, k, m2 - 1, m3) * 8 + (c)]
+#define G3(c, i, j, k) gf3[IX3(i, j, k, m2, m3 - 1) * 8 + (c)]
+    for (int i = 0; i < m1; ++i)
+        for (int j = 0; j < m2 - 1; ++j)
+            for (int k = 0; k < m3 - 1; ++k) {
+                double a = 0.5 * (G2(5, i, j, k) + G2(5, i, j, k + 1));
+                double b = 0.5 * (G3(5, i, j, k) + G3(5, i, j + 1, k));
+                ee1[IX3(i, j, k, m2 - 1, m3 - 1)] = 0.5 * (a + b);
+            }
+    for (int i = 0; i < m1 - 1; ++i)
+        for (int j = 0; j < m2; ++j)
+            for (int k = 0; k < m3 - 1; ++k) {
+                double a = 0.5 * (G3(6, i, j, k) + G3(6, i + 1, j, k));
+                double b = 0.5 * (G1(6, i, j, k) + G1(6, i, j, k + 1));
+                ee2[IX3(i, j, k, m2, m3 - 1)] = 0.5 * (a + b);
+            }
+    for (int i = 0; i < m1 - 1; ++i)
+        for (int j = 0; j < m2 - 1; ++j)
+            for (int k = 0; k < m3; ++k) {
+                double a = 0.5 * (G1(7, i, j, k) + G1(7, i, j + 1, k));
+                double b = 0.5 * (G2(7, i, j, k) + G2(7, i + 1, j, k));
+                ee3[IX3(i, j, k, m2 - 1, m3)] = 0.5 * (a + b);
+            }
+#undef G1
+#undef G2
+#undef G3
+}
+
+/* updated_conserved_density :220-241: uc (n1, n2, n3); gf extended by one on
+ * the transverse axes, gf1 (n1+1, n2+2, n3+2) etc. (remove_transverse_*,
+ * src/mesh_cartesian_3d.hpp:421-452); U - div(F) dt / dl with
+ * divergence_difference (:290-301). */
+void orc_mhd_v2_updated_conserved_density(int n1, int n2, int n3, const double* uc, const double* gf1,
+                                          const double* gf2, const double* gf3, double dt, double dl, double* out)
+{
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j)
+            for (int k = 0; k < n3; ++k) {
+                long c = IX3(i, j, k, n2, n3);
+                for (int q = 0; q < 5; ++q) {
+                    double d1 = gf1[IX3(i + 1, j + 1, k + 1, n2 + 2, n3 + 2) * 8 + q] - gf1[IX3(i, j + 1, k + 1, n2 + 2, n3 + 2) * 8 + q];
+                    double d2 = gf2[IX3(i + 1, j + 1, k + 1, n2 + 1, n3 + 2) * 8 + q] - gf2[IX3(i + 1, j, k + 1, n2 + 1, n3 + 2) * 8 + q];
+                    double d3 = gf3[IX3(i + 1, j + 1, k + 1, n2 + 2, n3 + 1) * 8 + q] - gf3[IX3(i + 1, j + 1, k, n2 + 2, n3 + 1) * 8 + q];
+                    out[c * 5 + q] = uc[c * 5 + q] - (d1 + d2 + d3) * dt / dl;
+                }
+            }
+}
+
+/* solenoidal_difference (src/mesh_cartesian_3d.hpp:265-276) of edge arrays
+ * e1 (n1, n2+1, n3+1), e2 (n1+1, n2, n3+1), e3 (n1+1, n2+1, n3) read through
+ * offsets (o1 on axis 0 of e1, o2 on axis 1 of e2, o3 on axis 2 of e3) with
+ * the stored extents t1, t2, t3 on those axes; each face value is
+ * bf - cf * dt / dl (bf NULL: 0 - ...) or, with bf == NULL and dt == 1 as the
+ * curl, cf * 1 / dl. */
+static void v2_faces_from_edges(int n1, int n2, int n3, const double* e1, const double* e2, const double* e3,
+                                int o, const double* bf1, const double* bf2, const double* bf3, double dt, double dl,
+                                int is_curl, double* b1, double* b2, double* b3)
+{
+    /* stored shapes: e1 (n1+2o, n2+1, n3+1), e2 (n1+1, n2+2o, n3+1), e3 (n1+1, n2+1, n3+2o) */
+#define E1(i, j, k) e1[IX3((i) + o, j, k, n2 + 1, n3 + 1)]
+#define E2(i, j, k) e2[IX3(i, (j) + o, k, n2 + 2 * o, n3 + 1)]
+#define E3(i, j, k) e3[IX3(i, j, (k) + o, n2 + 1, n3 + 2 * o)]
+    for (int i = 0; i <= n1; ++i)
+        for (int j = 0; j < n2; ++j)
+            for (int k = 0; k < n3; ++k) {
+                long f = IX3(i, j, k, n2, n3);
+                double cf = (E3(i, j + 1, k) - E3(i, j, k)) - (E2(i, j, k + 1) - E2(i, j, k));
+                b1[f] = is_curl ? cf * dt / dl : bf1[f] - cf * dt / dl;
+            }
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j <= n2; ++j)
+            for (int k = 0; k < n3; ++k) {
+                long f = IX3(i, j, k, n2 + 1, n3);
+                double cf = (E1(i, j, k + 1) - E1(i, j, k)) - (E3(i + 1, j, k) - E3(i, j, k));
+                b2[f] = is_curl ? cf * dt / dl : bf2[f] - cf * dt / dl;
+            }
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j)
+            for (int k = 0; k <= n3; ++k) {
+                long f = IX3(i, j, k, n2, n3 + 1);
+                double cf = (E2(i + 1, j, k) - E2(i, j, k)) - (E1(i, j + 1, k) - E1(i, j, k));
+                b3[f] = is_curl ? cf * dt / dl : bf3[f] - cf * dt / dl;
+            }
+#undef E1
+#undef E2
+#undef E3
+}
+
+/* curl :247-258: faces of cells (n1, n2, n3) from edges e1 (n1, n2+1, n3+1),
+ * e2 (n1+1, n2, n3+1), e3 (n1+1, n2+1, n3), over a unit time. */
+void orc_mhd_v2_curl(int n1, int n2, int n3, const double* e1, const double* e2, const double* e3, double dl,
+                     double* b1, double* b2, double* b3)
+{
+    v2_faces_from_edges(n1, n2, n3, e1, e2, e3, 0, NULL, NULL, NULL, 1.0, dl, 1, b1, b2, b3);
+}
+
+/* updated_magnetic_flux_density :264-278: edges as electromotive_forces
+ * returns them for gf extended by one (ee1 (n1+2, n2+1, n3+1), ...), the
+ * longitudinal end layers dropped (rl1..rl3 :67-69). */
+void orc_mhd_v2_updated_magnetic_flux_density(int n1, int n2, int n3, const double* bf1, const double* bf2,
+                                              const double* bf3, const double* ee1, const double* ee2,
+                                              const double* ee3, double dt, double dl, double* b1, double* b2,
+                                              double* b3)
+{
+    v2_faces_from_edges(n1, n2, n3, ee1, ee2, ee3, 1, bf1, bf2, bf3, dt, dl, 0, b1, b2, b3);
+}
+
+/* extend_cell_primitive_variables :126-135: the 27 blocks of a 3 x 3 x 3
+ * neighbourhood (row-major over the offsets, tile_blocks
+ * src/mesh_cartesian_3d.hpp:307-337) of bs^3 cells x nq components, cut to
+ * (bs + 2 count)^3 around the centre block (remove_surface :410-419). */
+void orc_mhd_v2_extend_cells(const double* const* blocks, int bs, int count, int nq, double* out)
+{
+    const int r = bs - count, m = bs + 2 * count;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j)
+            for (int k = 0; k < m; ++k) {
+                int I = i + r, J = j + r, K = k + r;
+                const double* b = blocks[(I / bs) * 9 + (J / bs) * 3 + K / bs];
+                memcpy(&out[IX3(i, j, k, m, m) * nq], &b[IX3(I % bs, J % bs, K % bs, bs, bs) * nq], sizeof(double) * nq);
+            }
+}
+
+/* extend_face_magnetic_flux_density :141-162: for axis a, the 9 face arrays
+ * blocks[9a .. 9a+8] of a 3 x 3 square normal to a (row-major over the two
+ * transverse offsets), tiled and cut to count zones on each transverse side
+ * (remove_transverse :458-464): bf1 (bs+1, bs+2c, bs+2c), etc. */
+void orc_mhd_v2_extend_faces(const double* const* blocks, int bs, int count, double* bf1, double* bf2, double* bf3)
+{
+    const int r = bs - count, m = bs + 2 * count;
+    for (int a = 0; a < 3; ++a) {
+        double* out = a == 0 ? bf1 : (a == 1 ? bf2 : bf3);
+        const int s1 = a == 0 ? bs + 1 : m, s2 = a == 1 ? bs + 1 : m, s3 = a == 2 ? bs + 1 : m;
+        const int b1 = a == 0 ? bs + 1 : bs, b2 = a == 1 ? bs + 1 : bs, b3 = a == 2 ? bs + 1 : bs;
+        for (int i = 0; i < s1; ++i)
+            for (int j = 0; j < s2; ++j)
+                for (int k = 0; k < s3; ++k) {
+                    int I = a == 0 ? i : i + r, J = a == 1 ? j : j + r, K = a == 2 ? k : k + r;
+                    /* block offsets on the axes, 0 on the normal axis (extent 1 there) */
+                    int oi = I / b1, oj = J / b2, ok = K / b3;
+                    int slot = a == 0 ? oj * 3 + ok : (a == 1 ? oi * 3 + ok : oi * 3 + oj);
+                    const double* b = blocks[9 * a + slot];
+                    out[IX3(i, j, k, s2, s3)] = b[IX3(I % b1, J % b2, K % b3, b2, b3)];
+                }
+    }
+}
+#undef IX3

@@ -6,7 +6,8 @@ oracle/Makefile compiles from the reference sources where they lie
 small .npz files of inputs and expected outputs; the reference itself never
 travels.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # all fixtures
+    python tests/golden/make_golden.py mhd_pcm    # only mhd_pcm.npz
 """
 import os
 import sys
@@ -174,6 +175,21 @@ def mhd_cases(rng):
                 uc_out=uo, bf1_out=o1, bf2_out=o2, bf3_out=o3, dt=dt)
 
 
+def mhd_pcm_cases():
+    """One advance_mhd_ct call with plm_theta = 0 (first-order face states,
+    src/scheme_mhd_v1.cpp:245-247,278 -> godunov_fluxes :119-131) on an 8^3
+    periodic cube: the composition of the mhd_scheme_v2 array functions
+    (src/scheme_mhd_v2.cpp) computes the same update."""
+    from mara3p_amd.problems import orszag_tang_3d
+    N = 8
+    uc, b1, b2, b3 = orszag_tang_3d(N, seed=5, perturbation=0.05)
+    ucl = np.ascontiguousarray(np.moveaxis(uc, 0, -1))
+    uo, o1, o2, o3, dt = np.zeros_like(ucl), np.zeros_like(b1), np.zeros_like(b2), np.zeros_like(b3), np.zeros(1)
+    st = oracle.ref().ref_mhd_advance_ct(N, P(ucl), P(b1), P(b2), P(b3), 1.0 / N, 0.33, 0.0, P(uo), P(o1), P(o2), P(o3), P(dt))
+    assert st == 0
+    return dict(N=N, uc=ucl, bf1=b1, bf2=b2, bf3=b3, uc_out=uo, bf1_out=o1, bf2_out=o2, bf3_out=o3, dt=dt)
+
+
 def srhd_cases(rng, m=300):
     """srhd::recover_primitive and riemann_solver (all four modes) vectors."""
     gam = rng.choice([4.0 / 3.0, 5.0 / 3.0], m)
@@ -197,6 +213,12 @@ def srhd_cases(rng, m=300):
 def main():
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libmara_ref.so is not built (make -C oracle ref)")
+    # fixtures with their own inputs can be rewritten alone: make_golden.py mhd_pcm
+    only = sys.argv[1:]
+    if "mhd_pcm" in only or not only:
+        np.savez_compressed(os.path.join(HERE, "mhd_pcm.npz"), **mhd_pcm_cases())
+    if only:
+        return
     rng = np.random.default_rng(SEED)
     np.savez_compressed(os.path.join(HERE, "plm.npz"), **plm_cases(rng))
     np.savez_compressed(os.path.join(HERE, "euler_pointwise.npz"), **euler_cases(rng))
