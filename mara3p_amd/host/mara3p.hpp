@@ -5,6 +5,7 @@
 //   modules::euler1d / euler2d      src/module_euler.hpp:77-125,161-207
 //   modules::locally_isothermal     src/module_locally_isothermal.hpp:73-117
 //   minidisk                        problems/minidisk.hpp:148-170
+//   mhd_scheme_v2                   src/scheme_mhd_v2.hpp:50-210
 //   mesh::block_index_t, next_on / prev_on   src/mesh_block_index.hpp:53-57,334-335
 //   mesh::cartesian_{1,2}d::geometry_t       src/mesh_geometry.hpp:53-101
 // Dimensional quantities (unit_time, unit_length, ...) are plain doubles
@@ -387,5 +388,200 @@ inline conserved_array_t updated_conserved(conserved_array_t uc, primitive_array
     return out;
 }
 } // namespace minidisk
+
+// --------------------------------------------------------------------------- mhd_scheme_v2
+namespace mhd {
+using primitive_t = std::array<double, 8>;           // rho, v1, v2, v3, p, B1, B2, B3
+using conserved_density_t = std::array<double, 5>;   // D, S1, S2, S3, E
+using godunov_data_t = std::array<double, 8>;        // 5 flux components, then the face electric field
+}
+
+// src/scheme_mhd_v2.hpp:50-210 (bodies src/scheme_mhd_v2.cpp:75-278): FP64,
+// adiabatic index 5/3.  Face and edge arrays hold the field values as doubles.
+namespace mhd_scheme_v2 {
+using cell_primitive_variables_t   = nd::shared_array<mhd::primitive_t, 3>;
+using cell_conserved_density_t     = nd::shared_array<mhd::conserved_density_t, 3>;
+using face_godunov_data_t          = std::array<nd::shared_array<mhd::godunov_data_t, 3>, 3>;
+using face_magnetic_flux_density_t = std::array<nd::shared_array<double, 3>, 3>;
+using edge_electromotive_density_t = std::array<nd::shared_array<double, 3>, 3>;
+
+namespace detail {
+using shape3 = nd::uivec_t<3>;
+inline std::size_t count(shape3 s) { return s[0] * s[1] * s[2]; }
+
+inline std::unique_ptr<device_buffer> up(const nd::shared_array<double, 3>& a)
+{
+    auto d = std::make_unique<device_buffer>(a.size());
+    hip_check(hipMemcpy(d->ptr, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return d;
+}
+
+template<std::size_t NC>
+inline std::unique_ptr<device_buffer> up(const nd::shared_array<std::array<double, NC>, 3>& a)
+{
+    auto d = std::make_unique<device_buffer>(a.size() * NC);
+    upload_soa(a.data(), a.size(), d->ptr);
+    return d;
+}
+
+inline nd::shared_array<double, 3> down(const device_buffer& d, shape3 s)
+{
+    nd::shared_array<double, 3> a(s);
+    hip_check(hipMemcpy(a.data(), d.ptr, count(s) * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return a;
+}
+
+template<std::size_t NC>
+inline nd::shared_array<std::array<double, NC>, 3> down_tuples(const device_buffer& d, shape3 s)
+{
+    nd::shared_array<std::array<double, NC>, 3> a(s);
+    download_soa(d.ptr, count(s), a.data());
+    return a;
+}
+
+inline shape3 grow(shape3 s, int d0, int d1, int d2) { return { s[0] + d0, s[1] + d1, s[2] + d2 }; }
+inline shape3 face_shape(shape3 n, int a) { return grow(n, a == 0, a == 1, a == 2); }
+inline shape3 edge_shape(shape3 n, int a) { return grow(n, a != 0, a != 1, a != 2); }
+
+inline void require(bool ok, const char* what)
+{
+    if (!ok) throw std::invalid_argument(std::string(what) + " (array shapes do not match)");
+}
+} // namespace detail
+
+// primitive_array, src/scheme_mhd_v2.cpp:75-81
+inline cell_primitive_variables_t primitive_array(cell_conserved_density_t uc, face_magnetic_flux_density_t bf)
+{
+    using namespace detail;
+    const shape3 n = uc.shape();
+    for (int a = 0; a < 3; ++a) require(bf[a].shape() == face_shape(n, a), "mhd_scheme_v2::primitive_array");
+    auto du = up(uc);
+    auto b1 = up(bf[0]), b2 = up(bf[1]), b3 = up(bf[2]);
+    device_buffer dp(8 * count(n));
+    check(m3p_mhd_v2_primitive_array(du->ptr, b1->ptr, b2->ptr, b3->ptr, int(n[0]), int(n[1]), int(n[2]), dp.ptr, nullptr));
+    return down_tuples<8>(dp, n);
+}
+
+// godunov_fluxes, :168-185 (pc extended by one zone on all axes, bf on the transverse axes)
+inline face_godunov_data_t godunov_fluxes(cell_primitive_variables_t pc, face_magnetic_flux_density_t bf)
+{
+    using namespace detail;
+    const shape3 m = pc.shape();
+    for (int a = 0; a < 3; ++a)
+        require(bf[a].shape() == grow(m, -(a == 0), -(a == 1), -(a == 2)), "mhd_scheme_v2::godunov_fluxes");
+    auto dp = up(pc);
+    auto b1 = up(bf[0]), b2 = up(bf[1]), b3 = up(bf[2]);
+    device_buffer g1(8 * bf[0].size()), g2(8 * bf[1].size()), g3(8 * bf[2].size());
+    check(m3p_mhd_v2_godunov_fluxes(dp->ptr, b1->ptr, b2->ptr, b3->ptr, int(m[0]), int(m[1]), int(m[2]),
+                                    g1.ptr, g2.ptr, g3.ptr, nullptr));
+    return { down_tuples<8>(g1, bf[0].shape()), down_tuples<8>(g2, bf[1].shape()), down_tuples<8>(g3, bf[2].shape()) };
+}
+
+// electromotive_forces, :191-214
+inline edge_electromotive_density_t electromotive_forces(face_godunov_data_t gf)
+{
+    using namespace detail;
+    const shape3 m = { gf[1].shape(0), gf[0].shape(1), gf[0].shape(2) };
+    for (int a = 0; a < 3; ++a)
+        require(gf[a].shape() == grow(m, -(a == 0), -(a == 1), -(a == 2)), "mhd_scheme_v2::electromotive_forces");
+    auto g1 = up(gf[0]), g2 = up(gf[1]), g3 = up(gf[2]);
+    std::array<shape3, 3> es = { grow(m, 0, -1, -1), grow(m, -1, 0, -1), grow(m, -1, -1, 0) };
+    device_buffer e1(count(es[0])), e2(count(es[1])), e3(count(es[2]));
+    check(m3p_mhd_v2_electromotive_forces(g1->ptr, g2->ptr, g3->ptr, int(m[0]), int(m[1]), int(m[2]),
+                                          e1.ptr, e2.ptr, e3.ptr, nullptr));
+    return { down(e1, es[0]), down(e2, es[1]), down(e3, es[2]) };
+}
+
+// updated_conserved_density, :220-241 (gf extended by one zone on the transverse axes)
+inline cell_conserved_density_t updated_conserved_density(cell_conserved_density_t uc, face_godunov_data_t gf,
+                                                          double dt, double dl)
+{
+    using namespace detail;
+    const shape3 n = uc.shape();
+    for (int a = 0; a < 3; ++a)
+        require(gf[a].shape() == grow(n, a == 0 ? 1 : 2, a == 1 ? 1 : 2, a == 2 ? 1 : 2),
+                "mhd_scheme_v2::updated_conserved_density");
+    auto du = up(uc);
+    auto g1 = up(gf[0]), g2 = up(gf[1]), g3 = up(gf[2]);
+    device_buffer dout(5 * count(n));
+    check(m3p_mhd_v2_updated_conserved_density(du->ptr, g1->ptr, g2->ptr, g3->ptr, int(n[0]), int(n[1]), int(n[2]),
+                                               dt, dl, dout.ptr, nullptr));
+    return down_tuples<5>(dout, n);
+}
+
+// curl, :247-258
+inline face_magnetic_flux_density_t curl(edge_electromotive_density_t ee, double dl)
+{
+    using namespace detail;
+    const shape3 n = { ee[0].shape(0), ee[1].shape(1), ee[2].shape(2) };
+    for (int a = 0; a < 3; ++a) require(ee[a].shape() == edge_shape(n, a), "mhd_scheme_v2::curl");
+    auto e1 = up(ee[0]), e2 = up(ee[1]), e3 = up(ee[2]);
+    device_buffer b1(count(face_shape(n, 0))), b2(count(face_shape(n, 1))), b3(count(face_shape(n, 2)));
+    check(m3p_mhd_v2_curl(e1->ptr, e2->ptr, e3->ptr, int(n[0]), int(n[1]), int(n[2]), dl, b1.ptr, b2.ptr, b3.ptr, nullptr));
+    return { down(b1, face_shape(n, 0)), down(b2, face_shape(n, 1)), down(b3, face_shape(n, 2)) };
+}
+
+// updated_magnetic_flux_density, :264-278 (ee as electromotive_forces returns it
+// for Godunov data extended by one zone)
+inline face_magnetic_flux_density_t updated_magnetic_flux_density(face_magnetic_flux_density_t bf,
+                                                                  edge_electromotive_density_t ee, double dt, double dl)
+{
+    using namespace detail;
+    const shape3 n = { bf[1].shape(0), bf[0].shape(1), bf[0].shape(2) };
+    for (int a = 0; a < 3; ++a) {
+        require(bf[a].shape() == face_shape(n, a), "mhd_scheme_v2::updated_magnetic_flux_density");
+        require(ee[a].shape() == grow(edge_shape(n, a), 2 * (a == 0), 2 * (a == 1), 2 * (a == 2)),
+                "mhd_scheme_v2::updated_magnetic_flux_density");
+    }
+    auto b1 = up(bf[0]), b2 = up(bf[1]), b3 = up(bf[2]);
+    auto e1 = up(ee[0]), e2 = up(ee[1]), e3 = up(ee[2]);
+    device_buffer o1(bf[0].size()), o2(bf[1].size()), o3(bf[2].size());
+    check(m3p_mhd_v2_updated_magnetic_flux_density(b1->ptr, b2->ptr, b3->ptr, e1->ptr, e2->ptr, e3->ptr,
+                                                   int(n[0]), int(n[1]), int(n[2]), dt, dl, o1.ptr, o2.ptr, o3.ptr, nullptr));
+    return { down(o1, bf[0].shape()), down(o2, bf[1].shape()), down(o3, bf[2].shape()) };
+}
+
+// extend_cell_primitive_variables, :126-135 (27 blocks in the order of mesh::neighbors_27)
+inline cell_primitive_variables_t extend_cell_primitive_variables(std::vector<cell_primitive_variables_t> pc_vector,
+                                                                  std::size_t block_size, std::size_t count)
+{
+    using namespace detail;
+    if (pc_vector.size() != 27) throw std::invalid_argument("mhd_scheme_v2::extend_cell_primitive_variables (need 27 blocks)");
+    std::vector<std::unique_ptr<device_buffer>> blocks;
+    const double* ptrs[27];
+    for (std::size_t m = 0; m < 27; ++m) {
+        require(pc_vector[m].shape() == shape3{ block_size, block_size, block_size }, "mhd_scheme_v2::extend_cell_primitive_variables");
+        blocks.push_back(up(pc_vector[m]));
+        ptrs[m] = blocks.back()->ptr;
+    }
+    const std::size_t e = block_size + 2 * count;
+    device_buffer out(8 * e * e * e);
+    check(m3p_mhd_v2_extend_cell_primitive_variables(ptrs, int(block_size), int(count), out.ptr, nullptr));
+    return down_tuples<8>(out, { e, e, e });
+}
+
+// extend_face_magnetic_flux_density, :141-162 (27 face triplets, three runs of 9
+// in the order of mesh::neighbors_9 on axes 0, 1, 2)
+inline face_magnetic_flux_density_t extend_face_magnetic_flux_density(std::vector<face_magnetic_flux_density_t> bf_vector,
+                                                                      std::size_t block_size, std::size_t count)
+{
+    using namespace detail;
+    if (bf_vector.size() != 27) throw std::invalid_argument("mhd_scheme_v2::extend_face_magnetic_flux_density (need 27 blocks)");
+    const shape3 n = { block_size, block_size, block_size };
+    std::vector<std::unique_ptr<device_buffer>> blocks;
+    const double* ptrs[27];
+    for (std::size_t m = 0; m < 27; ++m) {
+        const int a = int(m / 9);
+        require(bf_vector[m][a].shape() == face_shape(n, a), "mhd_scheme_v2::extend_face_magnetic_flux_density");
+        blocks.push_back(up(bf_vector[m][a]));
+        ptrs[m] = blocks.back()->ptr;
+    }
+    const std::size_t e = block_size + 2 * count;
+    std::array<shape3, 3> os = { shape3{ block_size + 1, e, e }, shape3{ e, block_size + 1, e }, shape3{ e, e, block_size + 1 } };
+    device_buffer o1(detail::count(os[0])), o2(detail::count(os[1])), o3(detail::count(os[2]));
+    check(m3p_mhd_v2_extend_face_magnetic_flux_density(ptrs, int(block_size), int(count), o1.ptr, o2.ptr, o3.ptr, nullptr));
+    return { down(o1, os[0]), down(o2, os[1]), down(o3, os[2]) };
+}
+} // namespace mhd_scheme_v2
 
 } // namespace mara3p

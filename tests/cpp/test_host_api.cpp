@@ -145,6 +145,85 @@ static void test_minidisk(std::mt19937& rng)
     EXPECT(err <= 1e-12, "minidisk::updated_conserved within 1e-12 of the oracle");
 }
 
+// mhd_scheme_v2 through the host API on a block of unequal extents against the
+// oracle's restatement (src/scheme_mhd_v2.cpp:75-278)
+static void test_mhd_v2(std::mt19937& rng)
+{
+    using namespace mhd_scheme_v2;
+    std::uniform_real_distribution<double> U(0.5, 2.0);
+    std::normal_distribution<double> N(0.0, 0.5);
+    const std::size_t n1 = 4, n2 = 3, n3 = 5, m1 = n1 + 2, m2 = n2 + 2, m3 = n3 + 2;
+    auto scalar = [&] (std::size_t a, std::size_t b, std::size_t c) {
+        nd::shared_array<double, 3> x(nd::uivec(a, b, c));
+        for (std::size_t z = 0; z < x.size(); ++z) x.data()[z] = N(rng);
+        return x;
+    };
+    auto prim = [&] (std::size_t a, std::size_t b, std::size_t c) {
+        cell_primitive_variables_t x(nd::uivec(a, b, c));
+        for (std::size_t z = 0; z < x.size(); ++z)
+            x.data()[z] = { U(rng), N(rng), N(rng), N(rng), U(rng), N(rng), N(rng), N(rng) };
+        return x;
+    };
+    // primitive_array
+    face_magnetic_flux_density_t bf = { scalar(n1 + 1, n2, n3), scalar(n1, n2 + 1, n3), scalar(n1, n2, n3 + 1) };
+    auto p0 = prim(n1, n2, n3);
+    cell_conserved_density_t uc(nd::uivec(n1, n2, n3));
+    for (std::size_t z = 0; z < uc.size(); ++z) orc_mhd_conserved_density(p0.data()[z].data(), 5.0 / 3.0, uc.data()[z].data());
+    auto pc = primitive_array(uc, bf);
+    std::vector<double> ref(8 * n1 * n2 * n3);
+    int st = orc_mhd_v2_primitive_array(int(n1), int(n2), int(n3), flat(uc).data(), bf[0].data(), bf[1].data(),
+                                        bf[2].data(), ref.data());
+    EXPECT(st == 0 && flat(pc) == ref, "mhd_scheme_v2::primitive_array equals the oracle exactly");
+    // godunov_fluxes on an extended block
+    auto pce = prim(m1, m2, m3);
+    face_magnetic_flux_density_t bfe = { scalar(m1 - 1, m2, m3), scalar(m1, m2 - 1, m3), scalar(m1, m2, m3 - 1) };
+    auto gf = godunov_fluxes(pce, bfe);
+    std::vector<double> r1(8 * bfe[0].size()), r2(8 * bfe[1].size()), r3(8 * bfe[2].size());
+    orc_mhd_v2_godunov_fluxes(int(m1), int(m2), int(m3), flat(pce).data(), bfe[0].data(), bfe[1].data(), bfe[2].data(),
+                              r1.data(), r2.data(), r3.data());
+    double err = 0, scale = 0;
+    for (auto [g, r] : { std::pair{ &gf[0], &r1 }, std::pair{ &gf[1], &r2 }, std::pair{ &gf[2], &r3 } }) {
+        auto v = flat(*g);
+        for (std::size_t z = 0; z < v.size(); ++z) { err = std::fmax(err, std::fabs(v[z] - (*r)[z])); scale = std::fmax(scale, std::fabs((*r)[z])); }
+    }
+    std::printf("mhd_scheme_v2::godunov_fluxes max relative error %.3e\n", err / scale);
+    EXPECT(err <= 1e-12 * scale, "mhd_scheme_v2::godunov_fluxes within 1e-12 of the oracle");
+    // electromotive_forces and updated_conserved_density, fed the same Godunov data
+    auto ee = electromotive_forces(gf);
+    std::vector<double> e1(ee[0].size()), e2(ee[1].size()), e3(ee[2].size());
+    auto f1 = flat(gf[0]), f2 = flat(gf[1]), f3 = flat(gf[2]);
+    orc_mhd_v2_electromotive_forces(int(m1), int(m2), int(m3), f1.data(), f2.data(), f3.data(), e1.data(), e2.data(), e3.data());
+    bool same = true;
+    for (std::size_t z = 0; z < e1.size(); ++z) same &= ee[0].data()[z] == e1[z];
+    for (std::size_t z = 0; z < e2.size(); ++z) same &= ee[1].data()[z] == e2[z];
+    for (std::size_t z = 0; z < e3.size(); ++z) same &= ee[2].data()[z] == e3[z];
+    EXPECT(same, "mhd_scheme_v2::electromotive_forces equals the oracle exactly");
+    auto uo = updated_conserved_density(uc, gf, 0.01, 0.2);
+    std::vector<double> ru(5 * uc.size());
+    orc_mhd_v2_updated_conserved_density(int(n1), int(n2), int(n3), flat(uc).data(), f1.data(), f2.data(), f3.data(),
+                                         0.01, 0.2, ru.data());
+    EXPECT(flat(uo) == ru, "mhd_scheme_v2::updated_conserved_density equals the oracle exactly");
+    auto bo = updated_magnetic_flux_density(bf, ee, 0.01, 0.2);
+    std::vector<double> b1(bf[0].size()), b2(bf[1].size()), b3(bf[2].size());
+    orc_mhd_v2_updated_magnetic_flux_density(int(n1), int(n2), int(n3), bf[0].data(), bf[1].data(), bf[2].data(),
+                                             e1.data(), e2.data(), e3.data(), 0.01, 0.2, b1.data(), b2.data(), b3.data());
+    same = true;
+    for (std::size_t z = 0; z < b1.size(); ++z) same &= bo[0].data()[z] == b1[z];
+    for (std::size_t z = 0; z < b2.size(); ++z) same &= bo[1].data()[z] == b2[z];
+    for (std::size_t z = 0; z < b3.size(); ++z) same &= bo[2].data()[z] == b3[z];
+    EXPECT(same, "mhd_scheme_v2::updated_magnetic_flux_density equals the oracle exactly");
+    // extend_cell_primitive_variables over 27 distinct blocks
+    std::vector<cell_primitive_variables_t> blocks;
+    std::vector<std::vector<double>> flats;
+    flats.reserve(27);
+    const double* ptrs[27];
+    for (int m = 0; m < 27; ++m) { blocks.push_back(prim(3, 3, 3)); flats.push_back(flat(blocks.back())); ptrs[m] = flats.back().data(); }
+    auto ext = extend_cell_primitive_variables(blocks, 3, 2);
+    std::vector<double> rx(8 * 7 * 7 * 7);
+    orc_mhd_v2_extend_cells(ptrs, 3, 2, 8, rx.data());
+    EXPECT(flat(ext) == rx, "mhd_scheme_v2::extend_cell_primitive_variables equals the oracle exactly");
+}
+
 int main()
 {
     std::mt19937 rng(1234);
@@ -152,6 +231,7 @@ int main()
     test_euler1d(rng);
     test_locally_isothermal(rng);
     test_minidisk(rng);
+    test_mhd_v2(rng);
     if (failures) { std::printf("%d FAILURES\n", failures); return 1; }
     std::printf("ALL PASS\n");
     return 0;
