@@ -24,6 +24,7 @@ template<typename R>
 struct EulerScheme {
     static constexpr int NC = 5;
     static constexpr int kWavesPerEU = is_fp64<R>::value ? 2 : 3;   // occupancy floor (VGPR budget)
+    static constexpr int kMarchUnroll = 2;   // rows per march iteration with a pipelined loader
     using Real = R;
     using Params = EulerParams;
     template<int H, int TJ> struct TileData {};
@@ -83,6 +84,9 @@ template<typename R>
 struct MinidiskScheme {
     static constexpr int NC = 3;
     static constexpr int kWavesPerEU = is_fp64<R>::value ? 2 : 4;
+    // three rows per iteration rotate the three-row window in registers; the
+    // FP64 kernel runs faster at two (register pressure)
+    static constexpr int kMarchUnroll = is_fp64<R>::value ? 2 : 3;
     using Real = R;
     using Params = MinidiskParams;
     // Vertex coordinates of the tile's rows/columns in FP64 (reference formula),
@@ -265,7 +269,7 @@ struct MinidiskScheme {
                 // 0.5 * rate * (1 + tanh(y)) = rate / (1 + exp(-2y))
                 br = float(prm.buffer_rate) * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * yb));
                 float ph = __builtin_amdgcn_rsqf(q0 + s2);
-                float vp = sqrtf(ph) - om * rr;
+                float vp = __builtin_amdgcn_sqrtf(ph) - om * rr;
                 ui[1] = vp * (-yf * ir0);
                 ui[2] = vp * (xf * ir0);
             }
@@ -295,6 +299,7 @@ template<typename R>
 struct IsoTableScheme {
     static constexpr int NC = 3;
     static constexpr int kWavesPerEU = 2;
+    static constexpr int kMarchUnroll = 2;
     using Real = R;
     using Params = IsoTableParams;
     template<int H, int TJ> struct TileData {};

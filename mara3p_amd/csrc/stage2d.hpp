@@ -105,7 +105,7 @@ struct MeshLoader {
     // Split load for software pipelining: fetch the raw conserved state of a
     // row zone one iteration ahead, recover it when it is needed.
     static constexpr bool kPrefetch = true;
-    static constexpr int kUnroll = 2;    // march unroll (fewer register rotations)
+    static constexpr bool kUnroll = true;    // march unrolled by the scheme's kMarchUnroll
     __device__ __forceinline__ void fetch_row(int b, int r, int c, double* raw) const { cons_row(b, r, c, raw); }
     __device__ __forceinline__ bool finish_row(int, int, int, const typename S::Params& prm, const double* raw,
                                                double* uu, typename S::Real* p) const
@@ -124,7 +124,7 @@ struct ExtendedLoader {
     const double* pe;
     int n;
     static constexpr bool kPrefetch = false;
-    static constexpr int kUnroll = 1;
+    static constexpr bool kUnroll = false;
     __device__ __forceinline__ void bind(int) {}
 
     // Block b of a list of blocks: uc + b NC n^2, pe + b NC (n+4)^2.
@@ -243,7 +243,7 @@ struct TreeLoader {
     // Software pipelining as MeshLoader for interior rows; a guard row is
     // assembled when it is needed.
     static constexpr bool kPrefetch = true;
-    static constexpr int kUnroll = 1;
+    static constexpr bool kUnroll = false;
     __device__ __forceinline__ void fetch_row(int b, int r, int c, double* raw) const
     {
         if (r >= 0 && r < n) cons_row(b, r, c, raw);
@@ -317,9 +317,9 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
     __shared__ R hraw[7][NC][H];     // pre-pass primitives at 7 edge columns
     __shared__ R hG[NC][H];          // y half slope at column c0-1
     __shared__ R hF[NC][H];          // y flux at face c0+TJ
-    __shared__ R rowP[NC][TJ + 2];   // [t+1] = column c0+t (edges come from hraw)
-    __shared__ R rowG[NC][TJ + 1];   // [t+1] = half slope at column c0+t (left edge from hG)
-    __shared__ R rowF[NC][TJ + 1];   // [t] = y flux at face c0+t (right edge from hF)
+    __shared__ R rowP[NC][TJ + 2];   // [t+1] = column c0+t; [0], [TJ+1] from hraw
+    __shared__ R rowG[NC][TJ + 1];   // [t+1] = half slope at column c0+t; [0] from hG
+    __shared__ R rowF[NC][TJ + 1];   // [t] = y flux at face c0+t; [TJ] from hF
 
     S::setup(td, prm, tc);
     bool ok = true;
@@ -405,7 +405,8 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
 
         double Unx[NC] = {};
         if constexpr (Ld::kPrefetch) ld.fetch_row(b, kfirst + 2 * D, c, Unx);
-#pragma unroll (Ld::kUnroll)
+        constexpr int kRowsPerIter = Ld::kUnroll ? S::kMarchUnroll : 1;
+#pragma unroll kRowsPerIter
         for (int kk = 0; kk < H; ++kk) {
             const int k = kfirst + D * kk;
             const int kl = k - tc.k0;
@@ -441,36 +442,46 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             }
             S::template face<1>(pl, pr, prm, td, tc, D > 0 ? k + 1 : k, c, Fa);
 
+            // The two edge lanes also store the out-of-tile neighbours of this
+            // row (from the pre-pass) at the ends of the LDS rows, so every
+            // lane reads its neighbours at the same offsets, with no select.
 #pragma unroll
             for (int m = 0; m < NC; ++m) rowP[m][t + 1] = Pa[m];
+            if (t == 0) {
+#pragma unroll
+                for (int m = 0; m < NC; ++m) { rowP[m][0] = hraw[1][m][kl]; rowG[m][0] = hG[m][kl]; }
+            }
+            if (t == TJ - 1) {
+#pragma unroll
+                for (int m = 0; m < NC; ++m) rowP[m][TJ + 1] = hraw[5][m][kl];
+            }
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                const R* left = t == 0 ? &hraw[1][m][kl] : &rowP[m][t];
-                const R* right = t == TJ - 1 ? &hraw[5][m][kl] : &rowP[m][t + 2];
-                gy[m] = plm_half(*left, Pa[m], *right, theta);
+                gy[m] = plm_half(rowP[m][t], Pa[m], rowP[m][t + 2], theta);
                 rowG[m][t + 1] = gy[m];
             }
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                const R* lp = t == 0 ? &hraw[1][m][kl] : &rowP[m][t];
-                const R* lg = t == 0 ? &hG[m][kl] : &rowG[m][t];
-                pl[m] = *lp + *lg;
+                pl[m] = rowP[m][t] + rowG[m][t];
                 pr[m] = Pa[m] - gy[m];
             }
             S::template face<2>(pl, pr, prm, td, tc, k, c, Fy);
 #pragma unroll
             for (int m = 0; m < NC; ++m) rowF[m][t] = Fy[m];
+            if (t == TJ - 1) {
+#pragma unroll
+                for (int m = 0; m < NC; ++m) rowF[m][TJ] = hF[m][kl];
+            }
             __syncthreads();
 
             double dfx[NC], dfy[NC], uu[NC], res[NC], fa[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                const R* rf = t == TJ - 1 ? &hF[m][kl] : &rowF[m][t + 1];
                 fa[m] = double(Fa[m]);
                 dfx[m] = D > 0 ? fa[m] - Fb[m] : Fb[m] - fa[m];
-                dfy[m] = double(*rf) - double(Fy[m]);
+                dfy[m] = double(rowF[m][t + 1]) - double(Fy[m]);
             }
             if constexpr (KEEPU) {
 #pragma unroll
