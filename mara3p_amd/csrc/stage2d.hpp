@@ -420,7 +420,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
                 if (a.u0) {
                     const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
-                    for (int m = 0; m < NC; ++m) U0[m] = z0[(size_t)m * n * n];
+                    for (int m = 0; m < NC; ++m) U0[m] = __builtin_nontemporal_load(&z0[(size_t)m * n * n]);
                 }
                 ok &= ld.finish_row(b, k + 2 * D, c, prm, raw, Uc, Pc);
             } else {
@@ -490,24 +490,27 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
                 ld.cons_row(b, k, c, uu);
             }
             S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, dl, res);
+            // The RK base and the output are touched once per launch:
+            // non-temporal loads and stores keep them from displacing the
+            // halo rows that neighbouring tiles share in L2.
             double* o = a.out + ((size_t)b * NC * n + k) * n + c;
             if (a.u0) {
                 if constexpr (!Ld::kPrefetch) {
                     const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
-                    for (int m = 0; m < NC; ++m) U0[m] = z0[(size_t)m * n * n];
+                    for (int m = 0; m < NC; ++m) U0[m] = __builtin_nontemporal_load(&z0[(size_t)m * n * n]);
                 }
                 if constexpr (is_fp64<R>::value) {
 #pragma unroll
-                    for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = U0[m] * a.rk_b + res[m] * (1.0 - a.rk_b);
+                    for (int m = 0; m < NC; ++m) __builtin_nontemporal_store(U0[m] * a.rk_b + res[m] * (1.0 - a.rk_b), &o[(size_t)m * n * n]);
                 } else {
                     const double wb = 1.0 - a.rk_b;
 #pragma unroll
-                    for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = fma(U0[m], a.rk_b, res[m] * wb);
+                    for (int m = 0; m < NC; ++m) __builtin_nontemporal_store(fma(U0[m], a.rk_b, res[m] * wb), &o[(size_t)m * n * n]);
                 }
             } else {
 #pragma unroll
-                for (int m = 0; m < NC; ++m) o[(size_t)m * n * n] = res[m];
+                for (int m = 0; m < NC; ++m) __builtin_nontemporal_store(res[m], &o[(size_t)m * n * n]);
             }
 #pragma unroll
             for (int m = 0; m < NC; ++m) { Pa[m] = Pb[m]; Pb[m] = Pc[m]; ga[m] = gb[m]; Fb[m] = fa[m]; }
