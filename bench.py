@@ -15,7 +15,7 @@ state resident in HBM.  A zone-update is one zone through one RK2 step
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload minidisk|kh|sedov_amr|ensemble|mhd]
 
 The other workloads are timed the same way for the record: kh = euler2d
-Kelvin-Helmholtz 4096^2; sedov_amr = euler2d Sedov blast on a 2:1 quad-tree
+Kelvin-Helmholtz 4096^2 with HLLC; sedov_amr = euler2d Sedov blast on a 2:1 quad-tree
 of 1408 leaves of 64^2 (levels 3-6); ensemble = 512 independent 1D Riemann
 problems of 512-8192 zones in one work-list launch per stage; mhd = ideal
 MHD with constrained transport, Orszag-Tang at 256^3 (the stage time step
@@ -87,9 +87,9 @@ def make_solver(workload, precision):
         s.zones = mesh.zones()
         return s, u0, (lambda t: dt), 5
     mesh, u0 = problems.kelvin_helmholtz(depth=4, block_size=256, seed=1)
-    s = Euler2DSolver(4, 256, gamma=5.0 / 3.0, plm_theta=1.5, precision=precision)
+    s = Euler2DSolver(4, 256, gamma=5.0 / 3.0, plm_theta=1.5, precision=precision, hllc=True)
     s.set_state(u0)
-    dt = 0.4 * s.dl / 2.0
+    dt = float(s.cfl_dt(0.4).item())
     s.zones = mesh.zones()
     return s, u0, (lambda t: dt), 5
 
@@ -155,7 +155,7 @@ def cpu_baseline(workload, u0, dt):
         u = np.ascontiguousarray(to_reference_layout(u0))
         nblk = u.shape[0]
         t0 = time.perf_counter()
-        orc().orc_euler2d_mesh_rk2(P(u), 16, 256, dt, 1.0 / 4096, 1.5, 5.0 / 3.0, 0, 1, threads)
+        orc().orc_euler2d_mesh_rk2(P(u), 16, 256, dt, 1.0 / 4096, 1.5, 5.0 / 3.0, 1, 1, threads)
         el = time.perf_counter() - t0
         zones = nblk * 256 * 256
         return {"value": zones / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
@@ -308,7 +308,7 @@ def main():
         result["config"] = {"workload": "minidisk iso2d+two_body, 16x16 blocks of 256^2 (16.8M zones), RK2, mixed",
                             "zones": zones, "blocks": 256, "block_size": 256}
     elif args.workload == "kh":
-        result["config"] = {"workload": "euler2d Kelvin-Helmholtz 4096^2 periodic (16x16 blocks of 256^2), PLM+HLLE, RK2",
+        result["config"] = {"workload": "euler2d Kelvin-Helmholtz 4096^2 periodic (16x16 blocks of 256^2), PLM+HLLC, RK2, CFL 0.4",
                             "zones": zones, "blocks": 256, "block_size": 256}
     elif args.workload == "sedov_amr":
         result["config"] = {"workload": "euler2d Sedov blast on a 2:1 quad-tree, 1408 leaves of 64^2 (levels 3-6), "
