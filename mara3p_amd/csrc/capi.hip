@@ -328,7 +328,7 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
     hipLaunchKernelGGL(mhd_prim_kernel<R>, dim3(std::min(gc, 4096)), dim3(256), 0, s, a, P);
     hipLaunchKernelGGL(mhd_dt_kernel, dim3(1), dim3(1), 0, s, a);
     FaceE<R> fe{ E1, E2, E3, N, nf };
-    if (!is_fp64<R>::value && N % 8 == 0 && N >= 8) {
+    if constexpr (!is_fp64<R>::value) if (N % 8 == 0 && N >= 8) {
         // column tiles: fluxes and the conserved update in one pass, then the
         // faces (the FP64 tile needs ~255 registers and runs slower than the
         // separate passes, so FP64 keeps those)

@@ -230,11 +230,33 @@ mhd_prim_kernel(MhdArgs a, R* P)
         double u[5], p[8];
 #pragma unroll
         for (int q = 0; q < 5; ++q) u[q] = a.uc[q * N3 + c];
-        int st = mhd_recover(u, b, a.gamma, p);
-        if (st) report_error(a.err, st == 1 ? 1 : 2, 0, (int)c);
+        if constexpr (is_fp64<R>::value) {
+            int st = mhd_recover(u, b, a.gamma, p);
+            if (st) report_error(a.err, st == 1 ? 1 : 2, 0, (int)c);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) P[q * N3 + c] = R(p[q]);
-        s = fmax(s, mhd_fastest_speed(p, a.gamma));
+            for (int q = 0; q < 8; ++q) P[q * N3 + c] = p[q];
+            s = fmax(s, mhd_fastest_speed(p, a.gamma));
+        } else {
+            // mixed: the pressure difference in FP64 (one reciprocal of the
+            // density instead of four divisions), the primitives stored and
+            // the CFL speed taken in FP32
+            const double d = u[0], id = 1.0 / d;
+            const double S2 = fma(u[3], u[3], fma(u[2], u[2], u[1] * u[1]));
+            const double B2 = fma(b[2], b[2], fma(b[1], b[1], b[0] * b[0]));
+            const double pg = (u[4] - 0.5 * S2 * id - 0.5 * B2) * (a.gamma - 1.0);
+            if (d < 0.0) report_error(a.err, 1, 0, (int)c);
+            else if (pg < 0.0) report_error(a.err, 2, 0, (int)c);
+            R pf[8] = { R(d), R(u[1] * id), R(u[2] * id), R(u[3] * id), R(pg), R(b[0]), R(b[1]), R(b[2]) };
+#pragma unroll
+            for (int q = 0; q < 8; ++q) P[q * N3 + c] = pf[q];
+            R m1, q1, m2, q2, m3, q3;
+            const R gf = R(a.gamma);
+            Mhd<R>::template wavespeeds<1>(pf, gf, m1, q1);
+            Mhd<R>::template wavespeeds<2>(pf, gf, m2, q2);
+            Mhd<R>::template wavespeeds<3>(pf, gf, m3, q3);
+            const R v = fmaxf(fmaxf(fmaxf(fabsf(m1), fabsf(q1)), fmaxf(fabsf(m2), fabsf(q2))), fmaxf(fabsf(m3), fabsf(q3)));
+            s = fmax(s, double(v));
+        }
     }
     s = wave_max(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
@@ -259,6 +281,15 @@ __global__ void mhd_dt_kernel(MhdArgs a)
 // primitive stencils of the three axes share the cell's neighbourhood, so
 // the primitives are read once from cache instead of once per axis.  Each
 // face's stencil: cells (m-2, m-1, m, m+1) along the axis for face m, wrapped.
+// Half the PLM slope of the middle value: the reference's 0.5 * plm_gradient
+// in FP64, the three-operand median form in FP32 (the same value).
+template<typename R>
+__device__ __forceinline__ R mhd_half_slope(R c0, R c1, R c2, R theta)
+{
+    if constexpr (is_fp64<R>::value) return R(0.5) * plm(c0, c1, c2, theta);
+    else return plm_half(c0, c1, c2, theta);
+}
+
 template<typename R, int A>
 __device__ __forceinline__ void mhd_one_face(const MhdArgs& a, const R* P, R* F, R* E, int i, int j, int k)
 {
@@ -284,8 +315,8 @@ __device__ __forceinline__ void mhd_one_face(const MhdArgs& a, const R* P, R* F,
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         R c0 = P[q * N3 + cell[0]], c1 = P[q * N3 + cell[1]], c2 = P[q * N3 + cell[2]], c3 = P[q * N3 + cell[3]];
-        pl[q] = c1 + R(0.5) * plm(c0, c1, c2, theta);
-        pr[q] = c2 - R(0.5) * plm(c1, c2, c3, theta);
+        pl[q] = c1 + mhd_half_slope(c0, c1, c2, theta);
+        pr[q] = c2 - mhd_half_slope(c1, c2, c3, theta);
     }
     const double* bf = A == 1 ? a.b1 : (A == 2 ? a.b2 : a.b3);
     R fl[5], e[3];
@@ -361,24 +392,32 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
     auto states = [&] (const R* c0, const R* c1, const R* c2, const R* c3, R* pl, R* pr) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            pl[q] = c1[q] + R(0.5) * plm(c0[q], c1[q], c2[q], theta);
-            pr[q] = c2[q] - R(0.5) * plm(c1[q], c2[q], c3[q], theta);
+            pl[q] = c1[q] + mhd_half_slope(c0[q], c1[q], c2[q], theta);
+            pr[q] = c2[q] - mhd_half_slope(c1[q], c2[q], c3[q], theta);
         }
     };
 
-    // x window: Pm = plane i-1, P0 = i, Pp = i+1; warm-up face at i0
-    R Pm[8], P0[8], Pp[8], Fx_lo[5];
+    // x window: Pm = plane i-1, P0 = i, Pp = i+1, and g0 = half slope at
+    // plane i (carried: the slope of the upper cell of one x face is the
+    // slope of the lower cell of the next); warm-up face at i0
+    R Pm[8], P0[8], Pp[8], g0[8], Fx_lo[5];
     {
         R Pm2[8], pl[8], pr[8], e[3];
         loadP(cell(i0 - 2, j, k), Pm2);
         loadP(cell(i0 - 1, j, k), Pm);
         loadP(cell(i0, j, k), P0);
         loadP(cell(i0 + 1, j, k), Pp);
-        states(Pm2, Pm, P0, Pp, pl, pr);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            g0[q] = mhd_half_slope(Pm[q], P0[q], Pp[q], theta);
+            pl[q] = Pm[q] + mhd_half_slope(Pm2[q], Pm[q], P0[q], theta);
+            pr[q] = P0[q] - g0[q];
+        }
         const long f = ((long)i0 * N + j) * N + k;
         Mhd<R>::template riemann<1>(pl, pr, R(a.b1[f]), gamma, Fx_lo, e);
         if (i0 == 0) { E1[f] = e[1]; E1[nf + f] = e[2]; }
     }
+    const double dtdl = dt / a.dl;
 
     for (int il = 0; il < TI; ++il) {
         const int i = i0 + il;
@@ -407,7 +446,13 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
         R Fx_hi[5];
         {
             R pl[8], pr[8], e[3];
-            states(Pm, P0, Pp, Pn, pl, pr);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const R gp = mhd_half_slope(P0[q], Pp[q], Pn[q], theta);
+                pl[q] = P0[q] + g0[q];
+                pr[q] = Pp[q] - gp;
+                g0[q] = gp;
+            }
             const long f = ((long)(i + 1) * N + j) * N + k;
             Mhd<R>::template riemann<1>(pl, pr, R(a.b1[f]), gamma, Fx_hi, e);
             E1[f] = e[1];
@@ -460,8 +505,8 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
                 double d1 = double(Fx_hi[q]) - double(Fx_lo[q]);
                 double d2 = double(F2s[q][jl + 1][kl]) - double(F2s[q][jl][kl]);
                 double d3 = double(F3s[q][jl][kl + 1]) - double(F3s[q][jl][kl]);
-                double r = a.uc[q * N3 + c] - (d1 + d2 + d3) * dt / a.dl;
-                if (a.u0) r = a.u0[q * N3 + c] * a.rk_b + r * (1.0 - a.rk_b);
+                double r = fma(-(d1 + d2 + d3), dtdl, a.uc[q * N3 + c]);
+                if (a.u0) r = fma(a.u0[q * N3 + c], a.rk_b, r * (1.0 - a.rk_b));
                 a.uo[q * N3 + c] = r;
             }
         }
