@@ -96,6 +96,39 @@ def make_solver(workload, precision):
 
 TILE_KIND = {"kh": 0, "minidisk": 1, "sedov_amr": 2}
 
+# conserved variables (the reference's component order) and the number of
+# steps of the mixed-vs-FP64 comparison per workload (minidisk: --l1-steps)
+CONSERVED = {"minidisk": ["Sigma", "Sx", "Sy"], "kh": ["D", "S1", "S2", "S3", "E"],
+             "sedov_amr": ["D", "S1", "S2", "S3", "E"], "ensemble": ["D", "S1", "S2", "S3", "E"]}
+L1_STEPS = {"kh": 200, "sedov_amr": 100, "ensemble": 100}
+
+
+def l1_report(workload, steps):
+    """Advance the FP64 and the mixed solver through the same time steps and
+    report, per conserved variable, mean |mixed - FP64| (absolute and relative
+    to mean |FP64|); periodic workloads also report the total mass of each
+    relative to the initial mass."""
+    a, ua, dtf, _ = make_solver(workload, "fp64")
+    b, _, _, _ = make_solver(workload, "mixed")
+    m0 = math.fsum(ua[:, 0].ravel())
+    for _ in range(steps):
+        dt = dtf(getattr(a, "time", 1.0))
+        a.step(dt)
+        b.step(dt)
+    a.check_errors()
+    b.check_errors()
+    ra, rb = a.state(), b.state()
+    per = {}
+    for q, name in enumerate(CONSERVED[workload]):
+        d = float(np.mean(np.abs(ra[:, q] - rb[:, q])))
+        per[name] = {"l1": d, "l1_rel": d / max(float(np.mean(np.abs(ra[:, q]))), 1e-300)}
+    rep = {"steps": steps, "per_variable": per}
+    if workload in ("minidisk", "kh"):
+        rep["mass_fp64"] = math.fsum(ra[:, 0].ravel()) / m0
+        rep["mass_mixed"] = math.fsum(rb[:, 0].ravel()) / m0
+    del a, b
+    return rep
+
 
 def tile_in_use(workload, precision):
     """The stage kernel's tile shape (columns, rows) from the library's table."""
@@ -220,7 +253,7 @@ def bench_mhd(args):
     from mara3p_amd.solvers import MhdSolver
     N = 256
     uc, b1, b2, b3 = problems.orszag_tang_3d(N, seed=1)
-    out = {}
+    out, final = {}, {}
     for precision in ("mixed", "fp64"):
         s = MhdSolver(N, precision=precision)
         s.set_state(uc, b1, b2, b3)
@@ -244,6 +277,8 @@ def bench_mhd(args):
         out[precision] = {"value": zones * args.steps / wall, "ms_per_step": 1e3 * wall / args.steps,
                           "event_ms_per_step": step_ms, "hbm_frac": alg / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                           "max_div_b": s.max_div_b()}
+        if args.l1_steps > 0:
+            final[precision] = s.state()
         del s
         torch.cuda.synchronize()
     m = out["mixed"]
@@ -255,6 +290,16 @@ def bench_mhd(args):
                            "unit": "GB/s", "frac": m["hbm_frac"], "traffic": None,
                            "note": "per RK2 step over all kernels of both stages"},
               "fp64": out["fp64"], "max_div_b_mixed": m["max_div_b"]}
+    if final:
+        # both precisions took warmup + steps RK2 steps, each with its own CFL time step per stage
+        names = ["D", "S1", "S2", "S3", "E", "B1", "B2", "B3"]
+        fa, fb = final["fp64"], final["mixed"]
+        arrays = [(fa[0][q], fb[0][q]) for q in range(5)] + [(fa[1 + a], fb[1 + a]) for a in range(3)]
+        per = {}
+        for name, (x, y) in zip(names, arrays):
+            d = float(np.mean(np.abs(x - y)))
+            per[name] = {"l1": d, "l1_rel": d / max(float(np.mean(np.abs(x))), 1e-300)}
+        result["l1_mixed_vs_fp64"] = {"steps": args.warmup + args.steps, "per_variable": per}
     print(json.dumps(result), flush=True)
 
 
@@ -331,20 +376,13 @@ def main():
         result["mixed_over_fp64"] = value / v64
         del s64
 
-    if args.l1_steps > 0 and args.workload == "minidisk":
-        a, ua, dtf, _ = make_solver("minidisk", "fp64")
-        b, _, _, _ = make_solver("minidisk", "mixed")
-        m0 = math.fsum(ua[:, 0].ravel())
-        for _ in range(args.l1_steps):
-            dt = dtf(a.time)
-            a.step(dt)
-            b.step(dt)
-        ra, rb = a.state(), b.state()
-        l1 = float(np.mean(np.abs(ra[:, 0] - rb[:, 0])))
-        result["l1_rho_mixed_vs_fp64"] = {"steps": args.l1_steps, "l1": l1,
-                                          "mass_fp64": math.fsum(ra[:, 0].ravel()) / m0,
-                                          "mass_mixed": math.fsum(rb[:, 0].ravel()) / m0}
-        del a, b
+    if args.l1_steps > 0:
+        steps = args.l1_steps if args.workload == "minidisk" else min(args.l1_steps, L1_STEPS[args.workload])
+        rep = l1_report(args.workload, steps)
+        result["l1_mixed_vs_fp64"] = rep
+        if args.workload == "minidisk":
+            result["l1_rho_mixed_vs_fp64"] = {"steps": steps, "l1": rep["per_variable"]["Sigma"]["l1"],
+                                              "mass_fp64": rep["mass_fp64"], "mass_mixed": rep["mass_mixed"]}
 
     if not args.no_cpu_baseline and args.workload in ("minidisk", "kh"):
         try:
