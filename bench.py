@@ -173,6 +173,32 @@ def timed_run(s, dt_of, steps, warmup, autotune=False):
 
 
 # ---------------------------------------------------------------------------
+def graph_run(s, steps, warmup, cfl=0.4):
+    """The stages captured in a hipGraph: one replay is two CFL-limited RK2
+    steps (device dt reduction, then both stages, twice).  Whole-step wall
+    time over the replays."""
+    import torch
+    from mara3p_amd.solvers import Graph, cfl_step_pair
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()       # capture needs a stream other than the default one
+    reps = max(1, steps // 2)
+    with torch.cuda.stream(stream):
+        with Graph() as g:
+            cfl_step_pair(s, cfl)
+        for _ in range(max(1, warmup // 2)):
+            g.replay()
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.replay()
+        stream.synchronize()
+        wall = time.perf_counter() - t0
+        s.check_errors()
+    torch.cuda.synchronize()
+    return {"value": s.zones * 2 * reps / wall, "ms_per_step": 1e3 * wall / (2 * reps), "steps": 2 * reps,
+            "note": "hipGraph replays of two CFL-limited RK2 steps (device CFL reduction + two stage launches each)"}
+
+
 def cpu_baseline(workload, u0, dt):
     """Reference CPU path on the host cores: the reference's own
     minidisk::updated_conserved (oracle/_ref) when built, else the oracle,
@@ -323,6 +349,7 @@ def main():
     zones = s.zones
     wall, d1, d2 = timed_run(s, dt_of, args.steps, args.warmup, args.autotune)
     tile = tile_in_use(args.workload, "mixed")
+    graph = graph_run(s, args.steps, args.warmup) if args.workload in ("kh", "sedov_amr", "ensemble") else None
     ms_per_step = 1e3 * wall / args.steps
     value = zones * args.steps / wall
     # algorithmic bytes: stage 1 reads U0 writes U1 (2 nc 8 B/zone), stage 2 reads U1, U0 writes U (3 nc 8 B/zone)
@@ -349,6 +376,8 @@ def main():
         "stage_ms": {"stage1": d1, "stage2": d2},
         "tile": tile,
     }
+    if graph:
+        result["graph"] = graph
     if args.workload == "minidisk":
         result["config"] = {"workload": "minidisk iso2d+two_body, 16x16 blocks of 256^2 (16.8M zones), RK2, mixed",
                             "zones": zones, "blocks": 256, "block_size": 256}
