@@ -175,28 +175,36 @@ def timed_run(s, dt_of, steps, warmup, autotune=False):
 # ---------------------------------------------------------------------------
 def graph_run(s, steps, warmup, cfl=0.4):
     """The stages captured in a hipGraph: one replay is two CFL-limited RK2
-    steps (device dt reduction, then both stages, twice).  Whole-step wall
-    time over the replays."""
+    steps.  Fused: each step's second stage also reduces the next step's CFL
+    time step (the library's *_stage_cfl entry points); separate: a CFL
+    reduction pass before each step.  Whole-step wall time over the replays."""
     import torch
-    from mara3p_amd.solvers import Graph, cfl_step_pair
-    torch.cuda.synchronize()
-    stream = torch.cuda.Stream()       # capture needs a stream other than the default one
+    from mara3p_amd.solvers import Graph, cfl_step_pair, cfl_step_pair_fused
     reps = max(1, steps // 2)
-    with torch.cuda.stream(stream):
-        with Graph() as g:
-            cfl_step_pair(s, cfl)
-        for _ in range(max(1, warmup // 2)):
-            g.replay()
-        stream.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            g.replay()
-        stream.synchronize()
-        wall = time.perf_counter() - t0
-        s.check_errors()
-    torch.cuda.synchronize()
-    return {"value": s.zones * 2 * reps / wall, "ms_per_step": 1e3 * wall / (2 * reps), "steps": 2 * reps,
-            "note": "hipGraph replays of two CFL-limited RK2 steps (device CFL reduction + two stage launches each)"}
+    out = {}
+    for name, pair in (("fused", cfl_step_pair_fused), ("separate", cfl_step_pair)):
+        torch.cuda.synchronize()
+        stream = torch.cuda.Stream()       # capture needs a stream other than the default one
+        with torch.cuda.stream(stream):
+            s.cfl_dt(cfl)
+            with Graph() as g:
+                pair(s, cfl)
+            for _ in range(max(1, warmup // 2)):
+                g.replay()
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                g.replay()
+            stream.synchronize()
+            wall = time.perf_counter() - t0
+            s.check_errors()
+        torch.cuda.synchronize()
+        del g
+        out[name] = wall
+    return {"value": s.zones * 2 * reps / out["fused"], "ms_per_step": 1e3 * out["fused"] / (2 * reps),
+            "separate_cfl_ms_per_step": 1e3 * out["separate"] / (2 * reps), "steps": 2 * reps,
+            "note": "hipGraph replays of two CFL-limited RK2 steps; value and ms_per_step with the CFL reduction "
+                    "fused into each step's second stage, separate_cfl_ms_per_step with a reduction pass per step"}
 
 
 def cpu_baseline(workload, u0, dt):

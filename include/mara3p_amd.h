@@ -266,9 +266,33 @@ int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, con
 int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
                           const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
                           double plm_theta, double gamma_law_index, int hllc, int precision, void* stream);
-/* CFL with a cell spacing per block: dt = cfl * min_b (dl_b / max speed). */
+/* CFL with a cell spacing per block: dt = cfl / max over zones of
+ * (max(|v1|, |v2|) + c_s) / dl_b (the reference fixes dt, src/mara.cpp:305-308;
+ * this time step is this library's). */
 int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, const double* dl_blk,
                             double gamma_law_index, double cfl, double* dt_out, void* stream);
+
+/* The three Euler stages above with the CFL reduction of their output fused
+ * in (no reference counterpart: a device step driver's CFL step without a
+ * separate pass over the state).  Same arguments and results as
+ * m3p_euler2d_stage / m3p_euler1d_stage_blocks / m3p_amr_euler2d_stage,
+ * and dt_next (device double) receives the time step m3p_euler_cfl_dt /
+ * m3p_euler_cfl_dt_blocks would compute from `out` with this cfl, bit for
+ * bit.  dt_next is written after the stage has read dt_dev, so the two may be
+ * the same buffer.  Shares the library's reduction scratch with the
+ * m3p_euler_cfl_dt* calls (calls on different streams must not overlap). */
+int m3p_euler2d_stage_cfl(const double* u, const double* u0, double* out, const int* nbr,
+                          int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
+                          double plm_theta, double gamma_law_index, int hllc, double cfl, double* dt_next,
+                          int precision, void* stream);
+int m3p_euler1d_stage_blocks_cfl(const double* u, const double* u0, double* out, const int* nbr, const double* dl_blk,
+                                 int nblk, int n, double dt, const double* dt_dev, double rk_b,
+                                 double plm_theta, double gamma_law_index, double cfl, double* dt_next,
+                                 int precision, void* stream);
+int m3p_amr_euler2d_stage_cfl(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
+                              const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
+                              double plm_theta, double gamma_law_index, int hllc, double cfl, double* dt_next,
+                              int precision, void* stream);
 
 /* hipGraph capture (no reference counterpart): wrap any sequence of
  * asynchronous m3p calls issued on `stream` between m3p_graph_begin and
@@ -281,10 +305,11 @@ int m3p_graph_launch(void* graph_exec, void* stream);
 int m3p_graph_destroy(void* graph_exec);
 
 /* Reductions --------------------------------------------------------------- */
-/* CFL time step for Euler states: dt = cfl * dl / max over zones of
- * (max(|v1|, |v2|) + c_s), written to dt_out (device double).  The reduction
- * is a wave64 shuffle max per workgroup and an atomic max across workgroups
- * (positive doubles order like their bit patterns), so it is deterministic. */
+/* CFL time step for Euler states: dt = cfl / max over zones of
+ * (max(|v1|, |v2|) + c_s) / dl, written to dt_out (device double).  The
+ * reduction is a wave64 shuffle max per workgroup and an atomic max across
+ * workgroups (positive doubles order like their bit patterns), so it is
+ * deterministic. */
 int m3p_euler_cfl_dt(const double* u, int nblk, long zones_per_block, double gamma_law_index,
                      double cfl, double dl, double* dt_out, void* stream);
 

@@ -85,33 +85,4 @@ amr_guard_fill_kernel(const double* u, double* pe, const int* recipes, int nleaf
     if (!ok) report_error(err, 1, b, (i * n + j));
 }
 
-// CFL with a cell spacing per block: dt = cfl * min_b (dl_b / max speed in b).
-__global__ void __launch_bounds__(256)
-euler_min_dl_over_speed_kernel(const double* u, int nblk, long zpb, const double* dl_blk, EulerParams prm,
-                               unsigned long long* slot)
-{
-    __shared__ double part[4];
-    double m = 1e300;
-    const long total = (long)nblk * zpb;
-    for (long z = (long)blockIdx.x * blockDim.x + threadIdx.x; z < total; z += (long)gridDim.x * blockDim.x) {
-        const long b = z / zpb, i = z % zpb;
-        const double* base = u + b * 5 * zpb + i;
-        double uu[5] = { base[0], base[zpb], base[2 * zpb], base[3 * zpb], base[4 * zpb] };
-        m = fmin(m, dl_blk[b] / Euler<double>::max_speed(uu, prm));
-    }
-    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, kWave));
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double v = part[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmin(v, part[w]);
-        atomicMin(slot, (unsigned long long)__double_as_longlong(v));
-    }
-}
-
-__global__ void cfl_from_min_kernel(const unsigned long long* slot, double cfl, double* dt_out)
-{
-    *dt_out = cfl * __longlong_as_double((long long)*slot);
-}
-
 } // namespace m3p

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include "mara3p_amd.h"
@@ -44,6 +45,9 @@ struct DeviceState {
 
 constexpr int kMaxDevices = 64;
 constexpr int kRedWords = 4096;
+// reduction scratch: words [0, 2048) totals partials, then single slots
+constexpr int kSlotDivB = 3000;
+constexpr int kSlotCflRate = 3008;           // kRateSlots words
 DeviceState g_dev[kMaxDevices];
 std::mutex g_mutex;
 
@@ -232,37 +236,38 @@ __global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long
     if (i < n) p[i] = v;
 }
 
+// CFL rate: max over zones of (max(|v1|, |v2|) + c_s) / dl_b, with the
+// reciprocal 1 / dl_b of the zone's block (dl_blk) or of the mesh (dl).  The
+// stage kernels with a fused reduction compute the same rate of the values
+// they write.  Positive doubles order like their bit patterns, so the
+// cross-workgroup atomic max is exact and deterministic.
 __global__ void __launch_bounds__(256)
-euler_max_speed_kernel(const double* u, int nblk, long zpb, EulerParams prm, unsigned long long* slot)
+euler_max_rate_kernel(const double* u, int nblk, long zpb, const double* dl_blk, double dl, EulerParams prm,
+                      unsigned long long* slots)
 {
-    __shared__ double part[4];
     double m = 0.0;
     const long total = (long)nblk * zpb;
     for (long z = (long)blockIdx.x * blockDim.x + threadIdx.x; z < total; z += (long)gridDim.x * blockDim.x) {
         const long b = z / zpb, i = z % zpb;
         const double* base = u + b * 5 * zpb + i;
         double uu[5] = { base[0], base[zpb], base[2 * zpb], base[3 * zpb], base[4 * zpb] };
-        m = fmax(m, Euler<double>::max_speed(uu, prm));
+        const double idl = 1.0 / (dl_blk ? dl_blk[b] : dl);
+        m = fmax(m, Euler<double>::max_speed(uu, prm) * idl);
     }
+    workgroup_atomic_max<256>(m, slots);
+}
+
+// dt = cfl / max rate (over the kRateSlots partial maxima; one wave)
+__global__ void __launch_bounds__(64) cfl_from_rate_kernel(const unsigned long long* slots, double cfl, double* dt_out)
+{
+    double m = __longlong_as_double((long long)slots[threadIdx.x]);
     m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double v = part[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, part[w]);
-        atomicMax(slot, (unsigned long long)__double_as_longlong(v));
-    }
+    if (threadIdx.x == 0) *dt_out = cfl / m;
 }
 
 __global__ void copy_u64_as_double_kernel(const unsigned long long* slot, double* out)
 {
     *out = __longlong_as_double((long long)*slot);
-}
-
-__global__ void cfl_finish_kernel(const unsigned long long* slot, double cfl, double dl, double* dt_out)
-{
-    double vmax = __longlong_as_double((long long)*slot);
-    *dt_out = cfl * dl / vmax;
 }
 
 // Two-level deterministic sum: per-workgroup partials (fixed order), then
@@ -454,9 +459,10 @@ int m3p_minidisk_updated_conserved(const double* uc, const double* pe, double* o
     return finish_blocking(stream);
 }
 
-int m3p_euler2d_stage(const double* u, const double* u0, double* out, const int* nbr,
-                      int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
-                      double plm_theta, double gamma_law_index, int hllc, int precision, void* stream)
+static int euler2d_stage(const double* u, const double* u0, double* out, const int* nbr,
+                         int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
+                         double plm_theta, double gamma_law_index, int hllc, int precision, void* stream,
+                         unsigned long long* rate_slot)
 {
     DeviceState* ds;
     if (int st = device_state(&ds)) return st;
@@ -465,11 +471,19 @@ int m3p_euler2d_stage(const double* u, const double* u0, double* out, const int*
     EulerParams prm{ gamma_law_index, plm_theta, hllc };
     StageArgs a{};
     a.u0 = u0; a.out = out; a.dt_dev = dt_dev; a.dt = dt; a.dl = dl; a.rk_b = rk_b;
-    a.n = n; a.nblk = nblk; a.err = ds->err;
+    a.n = n; a.nblk = nblk; a.err = ds->err; a.rate_slot = rate_slot;
     hipStream_t s = (hipStream_t)stream;
     if (precision == M3P_FP64)
         return launch_stage2d<EulerScheme<double>>(M3P_KIND_EULER2D, prm, MeshLoader<EulerScheme<double>>{u, nbr, n}, a, s);
     return launch_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, MeshLoader<EulerScheme<float>>{u, nbr, n}, a, s);
+}
+
+int m3p_euler2d_stage(const double* u, const double* u0, double* out, const int* nbr,
+                      int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
+                      double plm_theta, double gamma_law_index, int hllc, int precision, void* stream)
+{
+    return euler2d_stage(u, u0, out, nbr, nblk, n, dt, dt_dev, dl, rk_b, plm_theta, gamma_law_index, hllc,
+                         precision, stream, nullptr);
 }
 
 int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int* nbr, const int* bcoord,
@@ -522,11 +536,13 @@ int m3p_euler_cfl_dt(const double* u, int nblk, long zones_per_block, double gam
     if (!u || !dt_out || nblk <= 0 || zones_per_block <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad cfl arguments");
     hipStream_t s = (hipStream_t)stream;
     EulerParams prm{ gamma_law_index, 0.0, 0 };
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, ds->red, 1, 0ull);
+    unsigned long long* slot = ds->red + kSlotCflRate;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, slot, kRateSlots, 0ull);
     const long total = (long)nblk * zones_per_block;
     int grid = (int)std::min<long>((total + 255) / 256, 2048);
-    hipLaunchKernelGGL(euler_max_speed_kernel, dim3(grid), dim3(256), 0, s, u, nblk, zones_per_block, prm, ds->red);
-    hipLaunchKernelGGL(cfl_finish_kernel, dim3(1), dim3(1), 0, s, ds->red, cfl, dl, dt_out);
+    hipLaunchKernelGGL(euler_max_rate_kernel, dim3(grid), dim3(256), 0, s, u, nblk, zones_per_block,
+                       (const double*)nullptr, dl, prm, slot);
+    hipLaunchKernelGGL(cfl_from_rate_kernel, dim3(1), dim3(kRateSlots), 0, s, slot, cfl, dt_out);
     return launch_status("m3p_euler_cfl_dt");
 }
 
@@ -624,7 +640,7 @@ int m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, i
     if (!bf1 || !bf2 || !bf3 || !out || N < 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_max_div_b: bad arguments");
     hipStream_t s = (hipStream_t)stream;
     const long N3 = (long)N * N * N;
-    unsigned long long* slot = ds->red + 3000;
+    unsigned long long* slot = ds->red + kSlotDivB;
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, slot, 1, 0ull);
     hipLaunchKernelGGL(mhd_divb_kernel, dim3((int)std::min<long>((N3 + 255) / 256, 4096)), dim3(256), 0, s, bf1, bf2, bf3, N, slot);
     hipLaunchKernelGGL(copy_u64_as_double_kernel, dim3(1), dim3(1), 0, s, slot, out);
@@ -673,9 +689,10 @@ int m3p_srhd_riemann_array(const double* pl, const double* pr, double* f, long n
     return M3P_OK;
 }
 
-int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, const int* nbr, const double* dl_blk,
-                             int nblk, int n, double dt, const double* dt_dev, double rk_b,
-                             double plm_theta, double gamma_law_index, int precision, void* stream)
+static int euler1d_stage_blocks(const double* u, const double* u0, double* out, const int* nbr, const double* dl_blk,
+                                int nblk, int n, double dt, const double* dt_dev, double rk_b,
+                                double plm_theta, double gamma_law_index, int precision, void* stream,
+                                unsigned long long* rate_slot)
 {
     DeviceState* ds;
     if (int st = device_state(&ds)) return st;
@@ -684,14 +701,23 @@ int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, con
     EulerParams prm{ gamma_law_index, plm_theta, 0 };
     Stage1DArgs a{};
     a.u = u; a.u0 = u0; a.out = out; a.nbr = nbr; a.dt = dt; a.dt_dev = dt_dev; a.dl_blk = dl_blk; a.rk_b = rk_b;
-    a.n = n; a.nblk = nblk; a.err = ds->err;
+    a.n = n; a.nblk = nblk; a.err = ds->err; a.rate_slot = rate_slot;
     hipStream_t s = (hipStream_t)stream;
     return precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s, ds->cus) : launch_stage1d<float>(prm, a, s, ds->cus);
 }
 
-int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
-                          const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
-                          double plm_theta, double gamma_law_index, int hllc, int precision, void* stream)
+int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, const int* nbr, const double* dl_blk,
+                             int nblk, int n, double dt, const double* dt_dev, double rk_b,
+                             double plm_theta, double gamma_law_index, int precision, void* stream)
+{
+    return euler1d_stage_blocks(u, u0, out, nbr, dl_blk, nblk, n, dt, dt_dev, rk_b, plm_theta, gamma_law_index,
+                                precision, stream, nullptr);
+}
+
+static int amr_euler2d_stage(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
+                             const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
+                             double plm_theta, double gamma_law_index, int hllc, int precision, void* stream,
+                             unsigned long long* rate_slot)
 {
     DeviceState* ds;
     if (int st = device_state(&ds)) return st;
@@ -702,7 +728,7 @@ int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double
     hipStream_t s = (hipStream_t)stream;
     StageArgs a{};
     a.u0 = u0; a.out = out; a.dt_dev = dt_dev; a.dt = dt; a.dl_blk = dl_leaf; a.rk_b = rk_b;
-    a.n = n; a.nblk = nleaf; a.err = ds->err;
+    a.n = n; a.nblk = nleaf; a.err = ds->err; a.rate_slot = rate_slot;
     if (!pe_work) {
         // fused: guard zones read through the recipes inside the stage kernel
         if (precision == M3P_FP64)
@@ -718,6 +744,64 @@ int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double
     return launch_stage2d<EulerScheme<float>>(M3P_KIND_AMR_EULER2D, prm, ExtendedLoader<EulerScheme<float>>{u, pe_work, n}, a, s);
 }
 
+int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
+                          const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
+                          double plm_theta, double gamma_law_index, int hllc, int precision, void* stream)
+{
+    return amr_euler2d_stage(u, u0, out, pe_work, recipes, dl_leaf, nleaf, n, dt, dt_dev, rk_b, plm_theta,
+                             gamma_law_index, hllc, precision, stream, nullptr);
+}
+
+// Stages with the CFL reduction of their output fused in: the rate slot is
+// cleared before the stage and turned into dt_next after it (stream order,
+// so dt_next may be the dt_dev the stage reads).
+static int cfl_stage(double cfl, double* dt_next, void* stream, const std::function<int(unsigned long long*)>& stage)
+{
+    DeviceState* ds;
+    if (int st = device_state(&ds)) return st;
+    if (!dt_next || !(cfl > 0.0)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: a *_stage_cfl call needs cfl > 0 and dt_next");
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* slot = ds->red + kSlotCflRate;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, slot, kRateSlots, 0ull);
+    if (int st = launch_status("fill_u64_kernel")) return st;
+    if (int st = stage(slot)) return st;
+    hipLaunchKernelGGL(cfl_from_rate_kernel, dim3(1), dim3(kRateSlots), 0, s, slot, cfl, dt_next);
+    return launch_status("cfl_from_rate_kernel");
+}
+
+int m3p_euler2d_stage_cfl(const double* u, const double* u0, double* out, const int* nbr,
+                          int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
+                          double plm_theta, double gamma_law_index, int hllc, double cfl, double* dt_next,
+                          int precision, void* stream)
+{
+    return cfl_stage(cfl, dt_next, stream, [&] (unsigned long long* slot) {
+        return euler2d_stage(u, u0, out, nbr, nblk, n, dt, dt_dev, dl, rk_b, plm_theta, gamma_law_index, hllc,
+                             precision, stream, slot);
+    });
+}
+
+int m3p_euler1d_stage_blocks_cfl(const double* u, const double* u0, double* out, const int* nbr, const double* dl_blk,
+                                 int nblk, int n, double dt, const double* dt_dev, double rk_b,
+                                 double plm_theta, double gamma_law_index, double cfl, double* dt_next,
+                                 int precision, void* stream)
+{
+    return cfl_stage(cfl, dt_next, stream, [&] (unsigned long long* slot) {
+        return euler1d_stage_blocks(u, u0, out, nbr, dl_blk, nblk, n, dt, dt_dev, rk_b, plm_theta, gamma_law_index,
+                                    precision, stream, slot);
+    });
+}
+
+int m3p_amr_euler2d_stage_cfl(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
+                              const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
+                              double plm_theta, double gamma_law_index, int hllc, double cfl, double* dt_next,
+                              int precision, void* stream)
+{
+    return cfl_stage(cfl, dt_next, stream, [&] (unsigned long long* slot) {
+        return amr_euler2d_stage(u, u0, out, pe_work, recipes, dl_leaf, nleaf, n, dt, dt_dev, rk_b, plm_theta,
+                                 gamma_law_index, hllc, precision, stream, slot);
+    });
+}
+
 int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, const double* dl_blk,
                             double gamma_law_index, double cfl, double* dt_out, void* stream)
 {
@@ -726,12 +810,12 @@ int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, con
     if (!u || !dl_blk || !dt_out || nblk <= 0 || zones_per_block <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad cfl arguments");
     hipStream_t s = (hipStream_t)stream;
     EulerParams prm{ gamma_law_index, 0.0, 0 };
-    unsigned long long* slot = ds->red + 3001;
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, slot, 1, (unsigned long long)__builtin_bit_cast(long long, 1e300));
+    unsigned long long* slot = ds->red + kSlotCflRate;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, slot, kRateSlots, 0ull);
     const long total = (long)nblk * zones_per_block;
     int grid = (int)std::min<long>((total + 255) / 256, 2048);
-    hipLaunchKernelGGL(euler_min_dl_over_speed_kernel, dim3(grid), dim3(256), 0, s, u, nblk, zones_per_block, dl_blk, prm, slot);
-    hipLaunchKernelGGL(cfl_from_min_kernel, dim3(1), dim3(1), 0, s, slot, cfl, dt_out);
+    hipLaunchKernelGGL(euler_max_rate_kernel, dim3(grid), dim3(256), 0, s, u, nblk, zones_per_block, dl_blk, 0.0, prm, slot);
+    hipLaunchKernelGGL(cfl_from_rate_kernel, dim3(1), dim3(kRateSlots), 0, s, slot, cfl, dt_out);
     return launch_status("m3p_euler_cfl_dt_blocks");
 }
 

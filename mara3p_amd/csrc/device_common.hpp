@@ -85,6 +85,41 @@ __device__ __forceinline__ void report_error(int* err, int code, int block, int 
     }
 }
 
+// Max of a non-negative double over a workgroup of NT threads (NT a multiple
+// of 64, or one partial wave), then one atomic max into one of kRateSlots
+// slots picked by the workgroup index (non-negative doubles order like their
+// bit patterns; spreading the atomics keeps thousands of workgroups that end
+// together from serialising on one address).  Every thread of the workgroup
+// must call it.
+constexpr int kRateSlots = 64;
+
+template<int NT>
+__device__ __forceinline__ void workgroup_atomic_max(double v, unsigned long long* slots)
+{
+    unsigned long long* slot = slots + (blockIdx.x & (kRateSlots - 1));
+    __shared__ double part[(NT + 63) / 64 > 1 ? (NT + 63) / 64 : 1];
+    __shared__ double lanes[NT < 64 ? NT : 1];
+    if constexpr (NT >= 64) {
+        for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = part[0];
+            for (int w = 1; w < NT / 64; ++w) m = fmax(m, part[w]);
+            atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+        }
+    } else {
+        // a partial wave: no shuffles across inactive lanes
+        lanes[threadIdx.x] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = lanes[0];
+            for (int i = 1; i < NT; ++i) m = fmax(m, lanes[i]);
+            atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
 // Wave64 reductions through DPP/permute shuffles.
 __device__ __forceinline__ double wave_max(double v)
 {

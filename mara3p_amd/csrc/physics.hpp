@@ -162,12 +162,15 @@ struct Euler {
     }
 
     // Largest |v_axis| + c_s over the two axes (used by the CFL reduction).
+    // CFL signal speed max(|v1|, |v2|) + c_s of a conserved state (one
+    // reciprocal of the density); the stage kernels that fuse the CFL
+    // reduction and the separate reduction evaluate the same expression.
     __device__ static __forceinline__ double max_speed(const double* u, const Params& prm)
     {
-        double D = u[0];
-        double v1 = u[1] / D, v2 = u[2] / D, v3 = u[3] / D;
+        double D = u[0], iD = 1.0 / D;
+        double v1 = u[1] * iD, v2 = u[2] * iD, v3 = u[3] * iD;
         double pg = (u[4] - 0.5 * ((v1 * u[1] + v2 * u[2]) + v3 * u[3])) * (prm.gamma - 1.0);
-        double cs = sqrt(prm.gamma * pg / D);
+        double cs = sqrt(prm.gamma * pg * iD);
         return fmax(fabs(v1), fabs(v2)) + cs;
     }
 };

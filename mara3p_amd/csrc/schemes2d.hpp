@@ -25,8 +25,13 @@ struct EulerScheme {
     static constexpr int NC = 5;
     static constexpr int kWavesPerEU = is_fp64<R>::value ? 2 : 3;   // occupancy floor (VGPR budget)
     static constexpr int kMarchUnroll = 2;   // rows per march iteration with a pipelined loader
+    static constexpr bool kCfl = true;       // stage kernels can fuse the CFL reduction
     using Real = R;
     using Params = EulerParams;
+    __device__ static __forceinline__ double cfl_speed(const double* u, const Params& prm)
+    {
+        return Euler<double>::max_speed(u, prm);
+    }
     template<int H, int TJ> struct TileData {};
 
     __device__ static __forceinline__ double theta(const Params& prm) { return prm.theta; }
@@ -87,6 +92,8 @@ struct MinidiskScheme {
     // three rows per iteration rotate the three-row window in registers; the
     // FP64 kernel runs faster at two (register pressure)
     static constexpr int kMarchUnroll = is_fp64<R>::value ? 2 : 3;
+    static constexpr bool kCfl = false;
+    __device__ static __forceinline__ double cfl_speed(const double*, const MinidiskParams&) { return 0.0; }
     using Real = R;
     using Params = MinidiskParams;
     // Vertex coordinates of the tile's rows/columns in FP64 (reference formula),
@@ -300,8 +307,10 @@ struct IsoTableScheme {
     static constexpr int NC = 3;
     static constexpr int kWavesPerEU = 2;
     static constexpr int kMarchUnroll = 2;
+    static constexpr bool kCfl = false;
     using Real = R;
     using Params = IsoTableParams;
+    __device__ static __forceinline__ double cfl_speed(const double*, const Params&) { return 0.0; }
     template<int H, int TJ> struct TileData {};
 
     // theta = 2 is hard-coded by the reference (src/module_locally_isothermal.cpp:195-196)

@@ -24,6 +24,9 @@ struct Stage1DArgs {
     // extended-array mode (per-block compatibility): pe [5][n+4] primitives
     const double* pe;
     int outflow_lower, outflow_upper;
+    // fused CFL reduction: atomic max of (max(|v1|, |v2|) + c_s) / dl of the
+    // values written, or nullptr
+    unsigned long long* rate_slot;
 };
 
 // Persistent work-list form: the grid is sized to the device and each
@@ -61,11 +64,13 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
     if (blockIdx.x < items) fetch(blockIdx.x, nxt);
 
     bool ok = true;
+    double rate = 0.0;                  // fused CFL reduction
     for (long item = blockIdx.x; item < items; item += gridDim.x) {
         const long it = slot(item);
         const int b = (int)(it / tiles);
         const int c0 = (int)(it % tiles) * TJ;
         const double dl = a.dl_blk ? a.dl_blk[b] : a.dl;
+        const double idl = a.rate_slot ? 1.0 / dl : 0.0;
         double uz[NC], U0[NC];
 #pragma unroll
         for (int q = 0; q < NC; ++q) uz[q] = nxt[q];
@@ -124,6 +129,7 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
 
         const int z = c0 + t;
         double* o = a.out + (size_t)b * NC * n + z;
+        double wv[NC];
         for (int q = 0; q < NC; ++q) {
             double dfx = double(F[q][t + 1]) - double(F[q][t]);
             double res;
@@ -131,10 +137,13 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
             else res = uz[q] - dfx * (dt / dl);
             if (a.u0) res = U0[q] * a.rk_b + res * (1.0 - a.rk_b);
             o[(size_t)q * n] = res;
+            wv[q] = res;
         }
+        if (a.rate_slot) rate = fmax(rate, Euler<double>::max_speed(wv, prm) * idl);
         if (!ok) { report_error(a.err, 1, b, z); ok = true; }
         __syncthreads();   // LDS is reused by the next item
     }
+    if (a.rate_slot) workgroup_atomic_max<TJ>(rate, a.rate_slot);
 }
 
 } // namespace m3p

@@ -275,6 +275,9 @@ struct StageArgs {
     int n;                 // block size
     int nblk;
     int* err;
+    // fused CFL reduction (schemes with kCfl): atomic max of the rate
+    // max(|v1|, |v2|) + c_s over dl of the values written, or nullptr
+    unsigned long long* rate_slot;
 };
 
 // KEEPU: keep each zone's conserved state in a three-row register window from
@@ -312,6 +315,8 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
     const double dl = a.dl_blk ? a.dl_blk[b] : a.dl;
     Ld ld = ld_arg;
     ld.bind(b);
+    double rate = 0.0;                                    // fused CFL reduction
+    const double idl = a.rate_slot ? 1.0 / dl : 0.0;
 
     __shared__ typename S::template TileData<H, TJ> td;
     __shared__ R hraw[7][NC][H];     // pre-pass primitives at 7 edge columns
@@ -494,6 +499,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             // non-temporal loads and stores keep them from displacing the
             // halo rows that neighbouring tiles share in L2.
             double* o = a.out + ((size_t)b * NC * n + k) * n + c;
+            double wv[NC];
             if (a.u0) {
                 if constexpr (!Ld::kPrefetch) {
                     const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
@@ -502,15 +508,20 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
                 }
                 if constexpr (is_fp64<R>::value) {
 #pragma unroll
-                    for (int m = 0; m < NC; ++m) __builtin_nontemporal_store(U0[m] * a.rk_b + res[m] * (1.0 - a.rk_b), &o[(size_t)m * n * n]);
+                    for (int m = 0; m < NC; ++m) wv[m] = U0[m] * a.rk_b + res[m] * (1.0 - a.rk_b);
                 } else {
                     const double wb = 1.0 - a.rk_b;
 #pragma unroll
-                    for (int m = 0; m < NC; ++m) __builtin_nontemporal_store(fma(U0[m], a.rk_b, res[m] * wb), &o[(size_t)m * n * n]);
+                    for (int m = 0; m < NC; ++m) wv[m] = fma(U0[m], a.rk_b, res[m] * wb);
                 }
             } else {
 #pragma unroll
-                for (int m = 0; m < NC; ++m) __builtin_nontemporal_store(res[m], &o[(size_t)m * n * n]);
+                for (int m = 0; m < NC; ++m) wv[m] = res[m];
+            }
+#pragma unroll
+            for (int m = 0; m < NC; ++m) __builtin_nontemporal_store(wv[m], &o[(size_t)m * n * n]);
+            if constexpr (S::kCfl) {
+                if (a.rate_slot) rate = fmax(rate, S::cfl_speed(wv, prm) * idl);
             }
 #pragma unroll
             for (int m = 0; m < NC; ++m) { Pa[m] = Pb[m]; Pb[m] = Pc[m]; ga[m] = gb[m]; Fb[m] = fa[m]; }
@@ -523,6 +534,9 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
     if (((rem / tiles_j) & 1) == 0) march(std::integral_constant<int, 1>{});
     else                            march(std::integral_constant<int, -1>{});
     if (!ok) report_error(a.err, 1, b, 0);
+    if constexpr (S::kCfl) {
+        if (a.rate_slot) workgroup_atomic_max<TJ>(rate, a.rate_slot);
+    }
 }
 
 } // namespace m3p

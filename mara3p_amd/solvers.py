@@ -92,7 +92,14 @@ class Euler2DSolver(_StateSolver):
         self.nbr = torch.as_tensor(self.mesh.neighbours(), device=self.device)
         self.dt_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
 
-    def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None):
+    def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None, cfl=None, dt_next=None):
+        """One stage; with cfl and dt_next (device double) the CFL time step of
+        `out` is reduced in the same launch and written to dt_next."""
+        if cfl is not None:
+            check(lib().m3p_euler2d_stage_cfl(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), self.mesh.nblk,
+                                              self.mesh.n, dt, _ptr(dt_dev), self.dl, rk_b, self.theta, self.gamma,
+                                              self.hllc, cfl, _ptr(dt_next), self.precision, _stream()))
+            return
         check(lib().m3p_euler2d_stage(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), self.mesh.nblk, self.mesh.n,
                                       dt, _ptr(dt_dev), self.dl, rk_b, self.theta, self.gamma, self.hllc,
                                       self.precision, _stream()))
@@ -208,7 +215,13 @@ class AmrEuler2DSolver(_StateSolver):
         self.dl = torch.as_tensor(self.dl_host, dtype=torch.float64, device=self.device)
         self.dt_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
 
-    def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None):
+    def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None, cfl=None, dt_next=None):
+        if cfl is not None:
+            check(lib().m3p_amr_euler2d_stage_cfl(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.pe), _ptr(self.recipes),
+                                                  _ptr(self.dl), self.nleaf, self.n, dt, _ptr(dt_dev), rk_b,
+                                                  self.theta, self.gamma, self.hllc, cfl, _ptr(dt_next),
+                                                  self.precision, _stream()))
+            return
         check(lib().m3p_amr_euler2d_stage(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.pe), _ptr(self.recipes),
                                           _ptr(self.dl), self.nleaf, self.n, dt, _ptr(dt_dev), rk_b, self.theta,
                                           self.gamma, self.hllc, self.precision, _stream()))
@@ -257,7 +270,12 @@ class Euler1DEnsembleSolver(_StateSolver):
         self.dl = torch.as_tensor(mesh.cell_spacing(), dtype=torch.float64, device=self.device)
         self.dt_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
 
-    def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None):
+    def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None, cfl=None, dt_next=None):
+        if cfl is not None:
+            check(lib().m3p_euler1d_stage_blocks_cfl(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), _ptr(self.dl),
+                                                     self.mesh.nblk, self.mesh.n, dt, _ptr(dt_dev), rk_b, self.theta,
+                                                     self.gamma, cfl, _ptr(dt_next), self.precision, _stream()))
+            return
         check(lib().m3p_euler1d_stage_blocks(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), _ptr(self.dl),
                                              self.mesh.nblk, self.mesh.n, dt, _ptr(dt_dev), rk_b, self.theta,
                                              self.gamma, self.precision, _stream()))
@@ -321,6 +339,19 @@ def cfl_step_pair(solver, cfl):
         solver.stage(a, None, u1, 0.0, dt_dev=dt)
         solver.stage(u1, a, b, 0.0, dt_dev=dt)
     solver.u, solver.u1, solver.u2 = u, u1, u2
+
+
+def cfl_step_pair_fused(solver, cfl):
+    """cfl_step_pair with each step's CFL reduction fused into its second
+    stage: solver.dt_dev must hold the time step of the current state
+    (solver.cfl_dt(cfl) once, before the first call or graph replay), and each
+    step leaves the next step's time step there.  Same results as
+    cfl_step_pair, without the separate pass over the state."""
+    u, u1, u2 = solver.u, solver.u1, solver.u2
+    dt = solver.dt_dev
+    for a, b in ((u, u2), (u2, u)):
+        solver.stage(a, None, u1, 0.0, dt_dev=dt)
+        solver.stage(u1, a, b, 0.0, dt_dev=dt, cfl=cfl, dt_next=dt)
 
 
 class MhdSolver:
