@@ -309,7 +309,7 @@ long mhd_workspace_bytes(int N, int precision)
 {
     const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
     const long rs = precision == M3P_FP64 ? 8 : 4;
-    return (8 * N3 + 3 * 7 * nf) * rs + 256;
+    return (8 * N3 + 3 * 7 * nf) * rs + 1024;    // + alignment, the speed slots and dt
 }
 
 template<typename R>
@@ -327,11 +327,11 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
     char* tail = reinterpret_cast<char*>(E3 + 2 * nf);
     tail += (256 - (reinterpret_cast<uintptr_t>(tail) & 255)) & 255;
     a.vmax = reinterpret_cast<unsigned long long*>(tail);
-    if (!a.dt) a.dt = reinterpret_cast<double*>(tail + 8);
+    if (!a.dt) a.dt = reinterpret_cast<double*>(tail + 8 * kRateSlots);
     const int gc = (int)((N3 + 255) / 256);
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, a.vmax, 1, 0ull);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, a.vmax, kRateSlots, 0ull);
     hipLaunchKernelGGL(mhd_prim_kernel<R>, dim3(std::min(gc, 4096)), dim3(256), 0, s, a, P);
-    hipLaunchKernelGGL(mhd_dt_kernel, dim3(1), dim3(1), 0, s, a);
+    hipLaunchKernelGGL(mhd_dt_kernel, dim3(1), dim3(kRateSlots), 0, s, a);
     FaceE<R> fe{ E1, E2, E3, N, nf };
     if constexpr (!is_fp64<R>::value) if (N % 8 == 0 && N >= 8) {
         // column tiles: fluxes and the conserved update in one pass, then the

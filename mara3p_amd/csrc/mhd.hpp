@@ -204,20 +204,18 @@ struct MhdArgs {
     int N;
     double dl, cfl, theta, gamma;
     double* dt;                      // device: CFL time step of this call (written by mhd_dt_kernel)
-    unsigned long long* vmax;        // device: max fastest speed (as ordered bits)
+    unsigned long long* vmax;        // device: kRateSlots partial maxima of the fastest speed (ordered bits)
     int* err;
 };
 
 __device__ __forceinline__ int wrapi(int i, int N) { return i < 0 ? i + N : (i >= N ? i - N : i); }
 
 // Grid-stride over cells with a bounded grid, so the speed reduction ends in
-// one atomic per workgroup of a few thousand (one per 256 cells would
-// serialise on the single slot).
+// one atomic per workgroup of a few thousand, spread over kRateSlots slots.
 template<typename R>
 __global__ void __launch_bounds__(256)
 mhd_prim_kernel(MhdArgs a, R* P)
 {
-    __shared__ double part[4];
     const int N = a.N;
     const long N3 = (long)N * N * N;
     const long NN = (long)N * N;
@@ -258,20 +256,14 @@ mhd_prim_kernel(MhdArgs a, R* P)
             s = fmax(s, double(v));
         }
     }
-    s = wave_max(s);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double v = part[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, part[w]);
-        atomicMax(a.vmax, (unsigned long long)__double_as_longlong(v));
-    }
+    workgroup_atomic_max<256>(s, a.vmax);
 }
 
-__global__ void mhd_dt_kernel(MhdArgs a)
+// dt from the kRateSlots partial maxima (one wave)
+__global__ void __launch_bounds__(64) mhd_dt_kernel(MhdArgs a)
 {
-    double vmax = __longlong_as_double((long long)*a.vmax);
-    *a.dt = a.cfl * a.dl / vmax;
+    double vmax = wave_max(__longlong_as_double((long long)a.vmax[threadIdx.x]));
+    if (threadIdx.x == 0) *a.dt = a.cfl * a.dl / vmax;
 }
 
 // All three face families in one pass: the thread of cell (i, j, k)
