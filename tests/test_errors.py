@@ -82,6 +82,14 @@ def test_stage_argument_errors():
     oc = torch.empty_like(uc)
     assert L.m3p_minidisk_updated_conserved(_p(uc), _p(pe), _p(oc), 1, 0, 0, 0.0, 1e-3, ctypes.byref(data),
                                             M3P_FP64, _s()) == M3P_ERR_BAD_ARGUMENT
+    # fused CFL stages need cfl > 0 and an output for the time step
+    dtn = torch.zeros(1, **f64)
+    assert L.m3p_euler2d_stage_cfl(_p(u), _p(None), _p(out), _p(nbr), 4, n, 1e-3, _p(None), 1 / n, 0.5, 1.5, 1.4, 0,
+                                   0.0, _p(dtn), M3P_FP64, _s()) == M3P_ERR_BAD_ARGUMENT
+    assert L.m3p_euler2d_stage_cfl(_p(u), _p(None), _p(out), _p(nbr), 4, n, 1e-3, _p(None), 1 / n, 0.5, 1.5, 1.4, 0,
+                                   0.4, _p(None), M3P_FP64, _s()) == M3P_ERR_BAD_ARGUMENT
+    assert L.m3p_euler2d_stage_cfl(_p(u), _p(None), _p(u), _p(nbr), 4, n, 1e-3, _p(None), 1 / n, 0.5, 1.5, 1.4, 0,
+                                   0.4, _p(dtn), M3P_FP64, _s()) == M3P_ERR_BAD_ARGUMENT
     # the library still works after rejected calls
     torch.cuda.synchronize()
     code = ctypes.c_int(0)
