@@ -207,6 +207,20 @@ def graph_run(s, steps, warmup, cfl=0.4):
                     "fused into each step's second stage, separate_cfl_ms_per_step with a reduction pass per step"}
 
 
+def host_cpu():
+    """The host CPU model and logical CPU count (from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(workload, u0, dt):
     """Reference CPU path on the host cores: the reference's own
     minidisk::updated_conserved (oracle/_ref) when built, else the oracle,
@@ -424,6 +438,7 @@ def main():
     if not args.no_cpu_baseline and args.workload in ("minidisk", "kh"):
         try:
             result["cpu_baseline"] = cpu_baseline(args.workload, u0, dt_of(1.0))
+            result["cpu_baseline"]["host"] = host_cpu()
         except Exception as e:  # the baseline is reported, never required
             result["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(result), flush=True)
