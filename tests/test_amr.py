@@ -230,19 +230,20 @@ def test_gpu_amr_guard_fill_matches_oracle():
 @pytest.mark.gpu
 @gpu
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_gpu_sedov_amr_mirror_symmetry(precision):
+@pytest.mark.parametrize("depth,n,steps", [(5, 16, 10), (6, 64, 4)])
+def test_gpu_sedov_amr_mirror_symmetry(precision, depth, n, steps):
     """The nested tree and the Sedov state are symmetric under x -> -x; so is
     the solution after CFL-limited steps through every kind of level jump.
-    In FP64 the fused and two-pass guard fills agree exactly."""
+    In FP64 the fused and two-pass guard fills agree exactly.  (6, 64) is the
+    Sedov configuration at its full size, 1408 leaves of 64^2."""
     from mara3p_amd.problems import sedov_amr
     from mara3p_amd.solvers import AmrEuler2DSolver
-    t = amr.nested_tree(5, 1.5)
-    n = 16
+    t = amr.nested_tree(depth, 1.5)
     s = AmrEuler2DSolver(t, n, precision=precision)
     s2 = AmrEuler2DSolver(t, n, precision=precision, fused=False)
     s.set_state(sedov_amr(t, n))
     s2.set_state(sedov_amr(t, n))
-    for _ in range(10):
+    for _ in range(steps):
         s.step_cfl(0.4)
         s2.step_cfl(0.4)
     s.check_errors()

@@ -55,20 +55,23 @@ gpu = pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a HIP devi
 
 @pytest.mark.gpu
 @gpu
-def test_gpu_ensemble_fp64_matches_oracle_per_problem():
+@pytest.mark.parametrize("count,lmin,lmax,seed,steps", [(24, 128, 2048, 3, 4), (512, 512, 8192, 1, 1)])
+def test_gpu_ensemble_fp64_matches_oracle_per_problem(count, lmin, lmax, seed, steps):
+    """Every problem of the ensemble equals the oracle's own chain run bit for
+    bit; (512, 512, 8192, 1) is the ensemble configuration at its full size."""
     from mara3p_amd.solvers import Euler1DEnsembleSolver
     n = 64
-    m, u = riemann_ensemble(count=24, block_size=n, lmin=128, lmax=2048, seed=3)
+    m, u = riemann_ensemble(count=count, block_size=n, lmin=lmin, lmax=lmax, seed=seed)
     s = Euler1DEnsembleSolver(m)
     s.set_state(u)
-    dt = 0.05 / 2048
-    for _ in range(4):
+    dt = 0.05 / 8192
+    for _ in range(steps):
         s.step(dt)
     s.check_errors()
     got = s.state()
-    for k in range(24):
+    for k in range(count):
         L = int(m.lengths[k])
-        ref = _oracle_problem(u[m.problem_slice(k)], L, n, dt, 4)
+        ref = _oracle_problem(u[m.problem_slice(k)], L, n, dt, steps)
         assert np.array_equal(got[m.problem_slice(k)], ref), k
 
 

@@ -132,8 +132,10 @@ def test_minidisk_updated_conserved_fp64(n, level, bi, bj):
 
 
 # --------------------------------------------------------------------------- whole-mesh stages
-@pytest.mark.parametrize("depth,n,hllc", [(1, 64, 0), (2, 32, 0), (1, 64, 1), (2, 256, 0)])
+@pytest.mark.parametrize("depth,n,hllc", [(1, 64, 0), (2, 32, 0), (1, 64, 1), (2, 256, 0), (4, 256, 1)])
 def test_euler2d_mesh_rk2_fp64_exact(depth, n, hllc):
+    """(4, 256, 1) is the Kelvin-Helmholtz configuration at its full size,
+    4096^2 with HLLC, one RK2 step."""
     from mara3p_amd import problems
     from mara3p_amd.mesh import to_device_layout, to_reference_layout
     from mara3p_amd.solvers import Euler2DSolver
@@ -263,6 +265,26 @@ def test_sod_to_t_0_2(precision):
         orc().orc_euler_exact_riemann(P(left), P(right), g, xi / 0.2, P(exact))
         ex[i] = exact[0]
     assert np.mean(np.abs(rho - ex)) < 3e-3
+
+
+def test_minidisk_full_size_rk2_fp64():
+    """The minidisk configuration at its full size (16 x 16 blocks of 256^2 on
+    [-8, 8]^2), one RK2 step: FP64 within 1e-12 of the oracle."""
+    from mara3p_amd import MinidiskData, problems
+    from mara3p_amd.mesh import to_device_layout, to_reference_layout
+    from mara3p_amd.solvers import MinidiskSolver
+    data = MinidiskData.defaults(block_size=256, depth=4, domain_radius=8.0, softening_length=0.05,
+                                 sink_radius=0.05, mach_number=10.0)
+    mesh, u0 = problems.minidisk(data, seed=1, noise=0.01)
+    s = MinidiskSolver(data)
+    s.set_state(u0)
+    dt = problems.minidisk_dt(data, 0.0)
+    s.step(dt)
+    s.check_errors()
+    ref = to_reference_layout(u0).copy()
+    assert orc().orc_minidisk_mesh_rk2(P(ref), OrcMinidisk.from_data(data), 0.0, dt, 1, 16) == 0
+    err = rel_err(s.state(), to_device_layout(ref))
+    assert err <= 1e-12, err
 
 
 # --------------------------------------------------------------------------- mixed precision

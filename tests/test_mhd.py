@@ -110,7 +110,7 @@ def _rel(a, b):
 
 @pytest.mark.gpu
 @gpu
-@pytest.mark.parametrize("N", [8, 16])
+@pytest.mark.parametrize("N", [8, 16, 128])
 def test_gpu_mhd_advance_ct_fp64_matches_oracle(N):
     from mara3p_amd.problems import orszag_tang_3d
     from mara3p_amd.solvers import MhdSolver
