@@ -145,8 +145,8 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBRARY_PATH):
             raise ImportError(
-                "mara3p_amd: HIP library %s is not built (run __graft_entry__.build() "
-                "or python mara3p_amd/build.py)" % LIBRARY_PATH)
+                "mara3p_amd: HIP library %s is not built (run python mara3p_amd/build.py, "
+                "which compiles it for gfx950 in-tree)" % LIBRARY_PATH)
         # The library shares device pointers and streams with PyTorch, so it
         # must bind to the same HIP runtime.  PyTorch ships its own
         # libamdhip64.so.7; loading torch first makes that the process's copy

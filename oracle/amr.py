@@ -1,5 +1,5 @@
 """Oracle: guard fill and euler2d stages on the leaves of a quad-tree with
-level jumps.  TEST INFRASTRUCTURE ONLY: tests load this as the checker; the
+level jumps.  TEST INFRASTRUCTURE ONLY: tests load this as the parity reference; the
 product path never imports it.
 
 The neighbour of each leaf is assembled as a whole virtual block, in the

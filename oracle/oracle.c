@@ -1,6 +1,6 @@
 /*
  * Oracle: CPU FP64 restatement of Mara3's Godunov block update.
- * TEST INFRASTRUCTURE ONLY -- the checker and the timed CPU baseline; the
+ * TEST INFRASTRUCTURE ONLY -- the parity reference of the tests and the timed CPU baseline; the
  * product path never calls it.  Build with -ffp-contract=off (no FMA) so that
  * every operation rounds exactly as the reference's x86-64 build does.
  *

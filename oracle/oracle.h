@@ -2,7 +2,7 @@
  * Oracle: a CPU FP64 restatement of Mara3's Godunov block update.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py load this library, and only as the checker
+ * cpu_baseline leg of bench.py load this library, and only as the parity reference
  * (or as the timed CPU baseline).  The product path (mara3p_amd) never calls
  * it and fails loudly when its HIP library is missing.
  *

@@ -1,4 +1,4 @@
-"""Pin the oracle (CPU restatement) before trusting it as the checker.
+"""Pin the oracle (CPU restatement) before trusting it as the parity reference.
 
 Each oracle function is checked against (i) golden vectors written from the
 reference's own code (tests/golden, make_golden.py), (ii) the live reference
