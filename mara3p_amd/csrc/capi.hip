@@ -212,6 +212,7 @@ MinidiskParams minidisk_params(const m3p_minidisk_data* d, double time)
     p.q_sink_cut = std::cbrt(104.0 * p.s6);
     p.buffer_rate = d->buffer_rate;
     p.buffer_scale = d->buffer_scale;
+    p.inv_buffer_scale = 1.0 / d->buffer_scale;
     double st[10];
     two_body_state(1.0, 1.0, d->mass_ratio, d->eccentricity, time, d->omega_frame, st);
     p.m1 = st[0]; p.x1 = st[1]; p.y1 = st[2];

@@ -198,8 +198,10 @@ struct Iso2d {
         }
     }
 
-    // riemann_hlle :156-173 with flux :139-151
-    template<int AXIS>
+    // riemann_hlle :156-173 with flux :139-151; RCP (FP64 only) divides once,
+    // by 1 / (a+ - a-) times each component, instead of dividing each
+    // component (within an ulp of the reference's form)
+    template<int AXIS, bool RCP = false>
     __device__ static __forceinline__ void hlle(const R* pl, const R* pr, R cs2, R* f)
     {
         if constexpr (is_fp64<R>::value) {
@@ -214,9 +216,16 @@ struct Iso2d {
             R arm = vnr - cs, arp = vnr + cs;
             R ap = smax(R(0), smax(alp, arp));
             R am = smin(R(0), smin(alm, arm));
+            if constexpr (RCP) {
+                const R inv = R(1) / (ap - am);
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
-                f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) / (ap - am);
+                for (int k = 0; k < 3; ++k)
+                    f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) * inv;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) / (ap - am);
+            }
         } else {
             R ul[3] = { pl[0], pl[0] * pl[1], pl[0] * pl[2] };
             R ur[3] = { pr[0], pr[0] * pr[1], pr[0] * pr[2] };

@@ -80,6 +80,7 @@ struct MinidiskParams {
     double q_sink_cut;   // (104 s6)^(1/3): beyond it exp(-q^3 / s6) is below 1e-45
     double buffer_rate;
     double buffer_scale;
+    double inv_buffer_scale;
     double m1, x1, y1;   // binary components at this stage's time
     double m2, x2, y2;
     int level;
@@ -107,7 +108,16 @@ struct MinidiskScheme {
     __device__ static __forceinline__ double theta(const Params&) { return 2.0; }
     __device__ static __forceinline__ bool recover(const double* u, const Params&, R* p)
     {
-        Iso2d<R>::recover(u, p);
+        if constexpr (is_fp64<R>::value) {
+            // one reciprocal of Sigma (within an ulp of the two divisions of
+            // recover_primitive, src/physics_iso2d.hpp:119-125)
+            const double is = 1.0 / u[0];
+            p[0] = u[0];
+            p[1] = u[1] * is;
+            p[2] = u[2] * is;
+        } else {
+            Iso2d<R>::recover(u, p);
+        }
         return true;
     }
 
@@ -172,6 +182,8 @@ struct MinidiskScheme {
             // M twice is folded into one product (within an ulp)
             double r2 = x * x + y * y;
             cs2 = (1.0 / sqrt(r2 + prm.s2)) * prm.inv_mach2;
+            Iso2d<R>::template hlle<AXIS, true>(pl, pr, cs2, f);
+            return;
         } else {
             float xf = AXIS == 1 ? td.xvf[fi - tc.k0] : td.xcf[fi - tc.k0];
             float yf = AXIS == 1 ? td.ycf[fj - tc.c0] : td.yvf[fj - tc.c0];
@@ -217,16 +229,19 @@ struct MinidiskScheme {
             // and then the initial state (initial_conserved_array :182-191)
             // does not enter
             double r = sqrt((0.0 + x * x) + y * y);
-            double yb = (r - prm.R) / prm.buffer_scale;
+            double yb = (r - prm.R) * prm.inv_buffer_scale;
             double br = 0.0;
             double ui[3] = { 1.0, 0.0, 0.0 };
             if (yb > -20.0) {
-                br = 0.5 * prm.buffer_rate * (1.0 + tanh(yb));
+                // 0.5 (1 + tanh y) = 1 / (1 + exp(-2 y)); reciprocals of the
+                // scale and of r0 in place of divisions (each within an ulp)
+                br = prm.buffer_rate / (1.0 + exp(-2.0 * yb));
                 double ph = -1.0 * 1.0 / sqrt((x * x + y * y) + prm.s2);
                 double r0 = sqrt(x * x + y * y);
                 double vp = sqrt(-ph) - om * r0;
-                ui[1] = 1.0 * (vp * (-y / r0));
-                ui[2] = 1.0 * (vp * (x / r0));
+                const double ir0 = 1.0 / r0;
+                ui[1] = 1.0 * (vp * (-y * ir0));
+                ui[2] = 1.0 * (vp * (x * ir0));
             }
             double sig = pc[0];
             double sg1[3] = { 0.0 / 1.0, sig * ax1, sig * ay1 };
@@ -270,7 +285,7 @@ struct MinidiskScheme {
             float q0 = fmaf(xf, xf, yf * yf);
             float ir0 = __builtin_amdgcn_rsqf(q0);
             float rr = q0 * ir0;
-            float yb = (rr - float(prm.R)) * float(1.0 / prm.buffer_scale);
+            float yb = (rr - float(prm.R)) * float(prm.inv_buffer_scale);
             float br = 0.0f, ui[3] = { 1.0f, 0.0f, 0.0f };
             if (yb > -20.0f) {
                 // 0.5 * rate * (1 + tanh(y)) = rate / (1 + exp(-2y))
