@@ -97,12 +97,16 @@ struct Mhd {
         const R* b = &p[5];
         R d = p[0];
         if constexpr (is_fp64<R>::value) {
+            // one reciprocal of the density for the three squared speeds,
+            // and bn * bn for pow(bn, 2) (each within an ulp of the
+            // reference's divisions and pow)
             R vn = R(0) + p[1 + a];
-            R cs2 = gamma * p[4] / d;
-            R ca2 = dot3(b, b) / d;
+            const R id = R(1) / d;
+            R cs2 = gamma * p[4] * id;
+            R ca2 = dot3(b, b) * id;
             R cf2 = cs2 + ca2;
             R bn = R(0) + b[a];
-            R can2 = pow(bn, R(2)) / d;
+            R can2 = (bn * bn) * id;
             R cg2 = sqrt(cf2 * cf2 - R(4) * cs2 * can2);
             R cf = sqrt(R(0.5) * (cf2 + cg2));
             wm = vn - cf;
@@ -152,10 +156,11 @@ struct Mhd {
         R am = smin(R(0), smin(alm, arm));
         R hi[3];
         if constexpr (is_fp64<R>::value) {
+            const R inv = R(1) / (ap - am);   // one division for the eight components
 #pragma unroll
-            for (int k = 0; k < 5; ++k) f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) / (ap - am);
+            for (int k = 0; k < 5; ++k) f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) * inv;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) hi[k] = (ap * il[k] - am * ir[k] - (pl[5 + k] - pr[5 + k]) * ap * am) / (ap - am);
+            for (int k = 0; k < 3; ++k) hi[k] = (ap * il[k] - am * ir[k] - (pl[5 + k] - pr[5 + k]) * ap * am) * inv;
         } else {
             R inv = frcp(ap - am), apam = ap * am;
 #pragma unroll
