@@ -161,7 +161,6 @@ struct Euler {
         else          hlle<AXIS>(pl, pr, prm, f);
     }
 
-    // Largest |v_axis| + c_s over the two axes (used by the CFL reduction).
     // CFL signal speed max(|v1|, |v2|) + c_s of a conserved state (one
     // reciprocal of the density); the stage kernels that fuse the CFL
     // reduction and the separate reduction evaluate the same expression.
