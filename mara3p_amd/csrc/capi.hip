@@ -98,8 +98,8 @@ TileShape pick_tile(int kind, int precision, int n)
     return best;
 }
 
-template<class S, class Ld>
-int launch_stage2d_shape(TileShape ts, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
+template<class S, class Ld, bool FIRST>
+int launch_stage2d_variant(TileShape ts, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
 {
     // The mixed kernels keep the conserved state in registers between its load
     // and its update; the FP64 kernels (register-bound) re-load it.
@@ -110,18 +110,28 @@ int launch_stage2d_shape(TileShape ts, const typename S::Params& prm, const Ld& 
         hipLaunchKernelGGL(kern, dim3(grid), dim3(ts.tj), 0, s, prm, ld, a);
         return launch_status("stage2d_kernel");
     };
-    if (ts.tj == 256 && ts.h == 128) return go(stage2d_kernel<S, Ld, 128, 256, K>);
-    if (ts.tj == 256 && ts.h == 64) return go(stage2d_kernel<S, Ld, 64, 256, K>);
-    if (ts.tj == 256 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 256, K>);
-    if (ts.tj == 256 && ts.h == 16) return go(stage2d_kernel<S, Ld, 16, 256, K>);
-    if (ts.tj == 128 && ts.h == 64) return go(stage2d_kernel<S, Ld, 64, 128, K>);
-    if (ts.tj == 128 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 128, K>);
-    if (ts.tj == 64 && ts.h == 64)  return go(stage2d_kernel<S, Ld, 64, 64, K>);
-    if (ts.tj == 64 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 64, K>);
-    if (ts.tj == 64 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 64, K>);
-    if (ts.tj == 32 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 32, K>);
-    if (ts.tj == 16 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 16, K>);
+    if (ts.tj == 256 && ts.h == 128) return go(stage2d_kernel<S, Ld, 128, 256, K, FIRST>);
+    if (ts.tj == 256 && ts.h == 64) return go(stage2d_kernel<S, Ld, 64, 256, K, FIRST>);
+    if (ts.tj == 256 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 256, K, FIRST>);
+    if (ts.tj == 256 && ts.h == 16) return go(stage2d_kernel<S, Ld, 16, 256, K, FIRST>);
+    if (ts.tj == 128 && ts.h == 64) return go(stage2d_kernel<S, Ld, 64, 128, K, FIRST>);
+    if (ts.tj == 128 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 128, K, FIRST>);
+    if (ts.tj == 64 && ts.h == 64)  return go(stage2d_kernel<S, Ld, 64, 64, K, FIRST>);
+    if (ts.tj == 64 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 64, K, FIRST>);
+    if (ts.tj == 64 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 64, K, FIRST>);
+    if (ts.tj == 32 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 32, K, FIRST>);
+    if (ts.tj == 16 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 16, K, FIRST>);
     return fail(M3P_ERR_BAD_SHAPE, "m3p: 2D block size must be 16, 32 or a multiple of 64 (got " + std::to_string(n) + ")");
+}
+
+template<class S, class Ld>
+int launch_stage2d_shape(TileShape ts, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
+{
+    // A stage without a base state has its own kernels only where the scheme
+    // fetches more rows ahead there (and the loader pipelines its rows).
+    if constexpr (S::kPrefetchRows > 1 && Ld::kPrefetch)
+        if (!a.u0) return launch_stage2d_variant<S, Ld, true>(ts, prm, ld, a, s);
+    return launch_stage2d_variant<S, Ld, false>(ts, prm, ld, a, s);
 }
 
 template<class S, class Ld>

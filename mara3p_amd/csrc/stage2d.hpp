@@ -282,8 +282,10 @@ struct StageArgs {
 
 // KEEPU: keep each zone's conserved state in a three-row register window from
 // its load to its update (no second read of the state); otherwise re-load it
-// at the update (fewer registers).
-template<class S, class Ld, int H, int TJ, bool KEEPU>
+// at the update (fewer registers).  FIRST: a stage without a base state
+// (a.u0 == nullptr is then assumed); it has fewer loads in flight per row, so
+// a scheme may ask it to fetch more rows ahead (S::kPrefetchRows).
+template<class S, class Ld, int H, int TJ, bool KEEPU, bool FIRST = false>
 __global__ void __launch_bounds__(TJ) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU, 8)))
 stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
 {
@@ -408,8 +410,15 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
         }
         __syncthreads();
 
-        double Unx[NC] = {};
-        if constexpr (Ld::kPrefetch) ld.fetch_row(b, kfirst + 2 * D, c, Unx);
+        // Prefetched raw rows: Unx = row k + 2D of the next iteration, Unx2 =
+        // the row after it when the stage prefetches two rows ahead.
+        constexpr int kAhead = FIRST ? S::kPrefetchRows : 1;
+        static_assert(kAhead == 1 || kAhead == 2, "one or two rows ahead");
+        double Unx[NC] = {}, Unx2[NC] = {};
+        if constexpr (Ld::kPrefetch) {
+            ld.fetch_row(b, kfirst + 2 * D, c, Unx);
+            if constexpr (kAhead == 2) ld.fetch_row(b, kfirst + 3 * D, c, Unx2);
+        }
         constexpr int kRowsPerIter = Ld::kUnroll ? S::kMarchUnroll : 1;
 #pragma unroll kRowsPerIter
         for (int kk = 0; kk < H; ++kk) {
@@ -421,8 +430,14 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
                 double raw[NC];
 #pragma unroll
                 for (int m = 0; m < NC; ++m) raw[m] = Unx[m];
-                if (kk + 1 < H) ld.fetch_row(b, k + 3 * D, c, Unx);
-                if (a.u0) {
+                if constexpr (kAhead == 2) {
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) Unx[m] = Unx2[m];
+                    if (kk + 2 < H) ld.fetch_row(b, k + 4 * D, c, Unx2);
+                } else {
+                    if (kk + 1 < H) ld.fetch_row(b, k + 3 * D, c, Unx);
+                }
+                if (!FIRST && a.u0) {
                     const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
                     for (int m = 0; m < NC; ++m) U0[m] = __builtin_nontemporal_load(&z0[(size_t)m * n * n]);
@@ -500,7 +515,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             // halo rows that neighbouring tiles share in L2.
             double* o = a.out + ((size_t)b * NC * n + k) * n + c;
             double wv[NC];
-            if (a.u0) {
+            if (!FIRST && a.u0) {
                 if constexpr (!Ld::kPrefetch) {
                     const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
