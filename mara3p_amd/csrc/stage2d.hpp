@@ -410,14 +410,14 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
         }
         __syncthreads();
 
-        // Prefetched raw rows: Unx = row k + 2D of the next iteration, Unx2 =
-        // the row after it when the stage prefetches two rows ahead.
+        // Prefetched raw rows: Unx[i] = row k + (2 + i) D of the iteration
+        // that follows, kAhead rows in flight.
         constexpr int kAhead = FIRST ? S::kPrefetchRows : 1;
-        static_assert(kAhead == 1 || kAhead == 2, "one or two rows ahead");
-        double Unx[NC] = {}, Unx2[NC] = {};
+        static_assert(kAhead >= 1 && kAhead <= H, "rows ahead within the tile");
+        double Unx[kAhead][NC] = {};
         if constexpr (Ld::kPrefetch) {
-            ld.fetch_row(b, kfirst + 2 * D, c, Unx);
-            if constexpr (kAhead == 2) ld.fetch_row(b, kfirst + 3 * D, c, Unx2);
+#pragma unroll
+            for (int i = 0; i < kAhead; ++i) ld.fetch_row(b, kfirst + (2 + i) * D, c, Unx[i]);
         }
         constexpr int kRowsPerIter = Ld::kUnroll ? S::kMarchUnroll : 1;
 #pragma unroll kRowsPerIter
@@ -429,14 +429,13 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             if constexpr (Ld::kPrefetch) {
                 double raw[NC];
 #pragma unroll
-                for (int m = 0; m < NC; ++m) raw[m] = Unx[m];
-                if constexpr (kAhead == 2) {
+                for (int m = 0; m < NC; ++m) raw[m] = Unx[0][m];
 #pragma unroll
-                    for (int m = 0; m < NC; ++m) Unx[m] = Unx2[m];
-                    if (kk + 2 < H) ld.fetch_row(b, k + 4 * D, c, Unx2);
-                } else {
-                    if (kk + 1 < H) ld.fetch_row(b, k + 3 * D, c, Unx);
-                }
+                for (int i = 0; i + 1 < kAhead; ++i)
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) Unx[i][m] = Unx[i + 1][m];
+                // the last row a tile reads is kfirst + (H + 1) D
+                if (kk + kAhead < H) ld.fetch_row(b, k + (2 + kAhead) * D, c, Unx[kAhead - 1]);
                 if (!FIRST && a.u0) {
                     const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
