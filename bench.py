@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--autotune", action="store_true",
                     help="time every tile shape first and keep the fastest (default: the library's tile table)")
     ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the hipGraph replays of CFL-limited steps (kh, sedov_amr, ensemble)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (for HBM counter calibration under rocprofv3)")
     return ap.parse_args()
@@ -335,8 +337,8 @@ def bench_mhd(args):
               "scaling": "weak", "vs_baseline": None, "dtype": "f32-flux/f64-state", "data": "synthetic",
               "config": {"workload": "ideal MHD CT, Orszag-Tang 256^3 periodic, PLM+HLLE, RK2", "zones": N ** 3},
               "roofline": {"bound": "hbm", "achieved": m["hbm_frac"] * HBM_PEAK_GBS, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": m["hbm_frac"], "traffic": None,
-                           "note": "per RK2 step over all kernels of both stages"},
+                           "unit": "GB/s", "frac": m["hbm_frac"], "traffic": traffic_from_profiles("mhd", "mixed"),
+                           "note": "per RK2 step over all kernels of both stages (achieved and traffic)"},
               "fp64": out["fp64"], "max_div_b_mixed": m["max_div_b"]}
     if final:
         # both precisions took warmup + steps RK2 steps, each with its own CFL time step per stage
@@ -358,20 +360,20 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
-    if args.workload == "mhd":
-        return bench_mhd(args)
-
     if args.copy_calibration:
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.workload == "mhd":
+        return bench_mhd(args)
     s, u0, dt_of, nc = make_solver(args.workload, "mixed")
     zones = s.zones
     wall, d1, d2 = timed_run(s, dt_of, args.steps, args.warmup, args.autotune)
     tile = tile_in_use(args.workload, "mixed")
-    graph = graph_run(s, args.steps, args.warmup) if args.workload in ("kh", "sedov_amr", "ensemble") else None
+    graph = (graph_run(s, args.steps, args.warmup)
+             if args.workload in ("kh", "sedov_amr", "ensemble") and not args.no_graph else None)
     ms_per_step = 1e3 * wall / args.steps
     value = zones * args.steps / wall
     # algorithmic bytes: stage 1 reads U0 writes U1 (2 nc 8 B/zone), stage 2 reads U1, U0 writes U (3 nc 8 B/zone)

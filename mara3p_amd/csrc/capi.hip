@@ -348,7 +348,10 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
         // column tiles: fluxes and the conserved update in one pass, then the
         // faces (the FP64 tile needs ~255 registers and runs slower than the
         // separate passes, so FP64 keeps those)
-        const int TI = N >= 64 ? 32 : N;
+        // 16-plane segments: 4096 workgroups at 256^3 (5.3 rounds over the
+        // CUs, against 2.7 at 32 planes) run 1 % faster despite the extra
+        // warm-up planes per segment
+        const int TI = N >= 64 ? 16 : N;
         const int segs = N / TI;
         if (N % 32 == 0) {
             const int grid = (N / 8) * (N / 32) * segs;
