@@ -219,6 +219,7 @@ MinidiskParams minidisk_params(const m3p_minidisk_data* d, double time)
     p.omega = d->omega_frame;
     p.sink_rate = d->sink_rate;
     p.s6 = std::pow(d->sink_radius, 6.0);
+    p.inv_s6 = 1.0 / p.s6;
     p.q_sink_cut = std::cbrt(104.0 * p.s6);
     p.buffer_rate = d->buffer_rate;
     p.buffer_scale = d->buffer_scale;

