@@ -352,9 +352,9 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
         // column tiles: fluxes and the conserved update in one pass, then the
         // faces (the FP64 tile needs ~255 registers and runs slower than the
         // separate passes, so FP64 keeps those)
-        // 16-plane segments: 4096 workgroups at 256^3 (5.3 rounds over the
-        // CUs, against 2.7 at 32 planes) run 1 % faster despite the extra
-        // warm-up planes per segment
+        // 16-plane segments: 4096 workgroups at 256^3, four full rounds of
+        // the 1024 resident at four waves per SIMD (32-plane segments run
+        // as fast; 16 x 16 column tiles 10 % slower)
         const int TI = N >= 64 ? 16 : N;
         const int segs = N / TI;
         if (N % 32 == 0) {
