@@ -48,20 +48,40 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
     __shared__ R G[NC][TJ + 2];   // [i] = half slope at zone c0 - 1 + i
     __shared__ R F[NC][TJ + 1];   // [i] = face c0 + i (left face of zone c0 + i)
 
-    // The lane's own zone of the next item is fetched while the current item
-    // is computed (software pipelining across the work list).
+    // The lane's own zone of the next item, and for lanes 0-3 the four guard
+    // zones of that item (c0 - 2, c0 - 1, c0 + TJ, c0 + TJ + 1: neighbour
+    // block, or the edge zone at an outflow boundary), are fetched while the
+    // current item is computed (software pipelining across the work list).
     // The second RK stage (u0 given) walks the list backwards: its first
     // reads are what the first stage wrote last (likeliest to be cached).
     const bool rev = a.u0 != nullptr;
     auto slot = [&] (long it) { return rev ? items - 1 - it : it; };
-    auto fetch = [&] (long it, double* raw) {
+    auto guard_zone = [&] (int b, int i) {
+        // zone index i in [-2, 0) or [n, n+2) of block b
+        int bb = b, z = i;
+        if (i < 0) {
+            int pb = a.nbr[2 * b];
+            if (pb < 0) z = 0; else { bb = pb; z = i + n; }
+        } else if (i >= n) {
+            int nb = a.nbr[2 * b + 1];
+            if (nb < 0) z = n - 1; else { bb = nb; z = i - n; }
+        }
+        return a.u + (size_t)bb * NC * n + z;
+    };
+    auto fetch = [&] (long it, double* raw, double* graw) {
         it = slot(it);
-        const double* src = a.u + (size_t)(it / tiles) * NC * n + (it % tiles) * TJ + t;
+        const int b = (int)(it / tiles), c0 = (int)(it % tiles) * TJ;
+        const double* src = a.u + (size_t)b * NC * n + c0 + t;
 #pragma unroll
         for (int q = 0; q < NC; ++q) raw[q] = src[(size_t)q * n];
+        if (!a.pe && t < 4) {
+            const double* g = guard_zone(b, t < 2 ? c0 - 2 + t : c0 + TJ + (t - 2));
+#pragma unroll
+            for (int q = 0; q < NC; ++q) graw[q] = g[(size_t)q * n];
+        }
     };
-    double nxt[NC] = {};
-    if (blockIdx.x < items) fetch(blockIdx.x, nxt);
+    double nxt[NC] = {}, gnxt[NC] = {};
+    if (blockIdx.x < items) fetch(blockIdx.x, nxt, gnxt);
 
     bool ok = true;
     double rate = 0.0;                  // fused CFL reduction
@@ -71,44 +91,30 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
         const int c0 = (int)(it % tiles) * TJ;
         const double dl = a.dl_blk ? a.dl_blk[b] : a.dl;
         const double idl = a.rate_slot ? 1.0 / dl : 0.0;
-        double uz[NC], U0[NC];
+        double uz[NC], gz[NC], U0[NC];
 #pragma unroll
-        for (int q = 0; q < NC; ++q) uz[q] = nxt[q];
-        if (item + gridDim.x < items) fetch(item + gridDim.x, nxt);
+        for (int q = 0; q < NC; ++q) { uz[q] = nxt[q]; gz[q] = gnxt[q]; }
+        if (item + gridDim.x < items) fetch(item + gridDim.x, nxt, gnxt);
         if (a.u0) {
 #pragma unroll
             for (int q = 0; q < NC; ++q) U0[q] = a.u0[(size_t)b * NC * n + (size_t)q * n + c0 + t];
         }
-        auto load = [&] (int i, R* p) {
-            // zone index i in [-2, n+2) of block b
-            if (a.pe) {
-                int e = i;
-                if (a.outflow_lower && i < 0) e = 0;
-                else if (!a.outflow_lower && a.outflow_upper && i >= n) e = n - 1;
-                for (int q = 0; q < NC; ++q) p[q] = R(a.pe[(size_t)q * (n + 4) + e + 2]);
-                return;
-            }
-            int bb = b, z = i;
-            if (i < 0) {
-                int pb = a.nbr[2 * b];
-                if (pb < 0) z = 0; else { bb = pb; z = i + n; }
-            } else if (i >= n) {
-                int nb = a.nbr[2 * b + 1];
-                if (nb < 0) z = n - 1; else { bb = nb; z = i - n; }
-            }
-            const double* src = a.u + (size_t)bb * NC * n + z;
-            double uu[NC];
-            for (int q = 0; q < NC; ++q) uu[q] = src[(size_t)q * n];
-            ok &= Euler<R>::recover(uu, prm, p);
+        // extended-array mode (per-block compatibility): zone index i in
+        // [-2, n+2) of the caller's primitive array
+        auto load_pe = [&] (int i, R* p) {
+            int e = i;
+            if (a.outflow_lower && i < 0) e = 0;
+            else if (!a.outflow_lower && a.outflow_upper && i >= n) e = n - 1;
+            for (int q = 0; q < NC; ++q) p[q] = R(a.pe[(size_t)q * (n + 4) + e + 2]);
         };
 
         R p[NC];
-        if (a.pe) load(c0 + t, p);
+        if (a.pe) load_pe(c0 + t, p);
         else ok &= Euler<R>::recover(uz, prm, p);
         for (int q = 0; q < NC; ++q) P[q][t + 2] = p[q];
         if (t < 4) {
-            int i = t < 2 ? c0 - 2 + t : c0 + TJ + (t - 2);
-            load(i, p);
+            if (a.pe) load_pe(t < 2 ? c0 - 2 + t : c0 + TJ + (t - 2), p);
+            else ok &= Euler<R>::recover(gz, prm, p);
             int slot = t < 2 ? t : TJ + 2 + (t - 2);
             for (int q = 0; q < NC; ++q) P[q][slot] = p[q];
         }
