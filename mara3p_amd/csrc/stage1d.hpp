@@ -4,8 +4,11 @@
 //   outflow boundary condition,           src/mara.cpp:123-143
 //
 // A workgroup of TJ threads owns TJ zones of one block.  Primitives of the TJ
-// zones and four guard zones go to LDS; slopes and face fluxes are computed
-// once each (TJ+2 slopes, TJ+1 faces).
+// zones and four guard zones go to LDS; each lane computes the right face of
+// its zone (from the half slopes at its zone and the next, the latter also
+// computed by the next lane, from the same operands), and the left faces of
+// TJ items at a time come from one pass of all lanes.  Every face flux is
+// computed once.
 #pragma once
 #include "physics.hpp"
 
@@ -44,9 +47,14 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
     const R theta = R(prm.theta);
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
 
+    // FP64 (bound by its arithmetic) computes the left faces of TJ items in
+    // one pass of all lanes; the mixed kernel (bound by memory latency) lets
+    // lane 0 compute its item's left face from the LDS row, which needs no
+    // further loads.
+    constexpr bool kGroupLeftFaces = is_fp64<R>::value;
     __shared__ R P[NC][TJ + 4];   // [i] = zone c0 - 2 + i
-    __shared__ R G[NC][TJ + 2];   // [i] = half slope at zone c0 - 1 + i
     __shared__ R F[NC][TJ + 1];   // [i] = face c0 + i (left face of zone c0 + i)
+    __shared__ R Fl[NC][TJ];      // left face of each item of the current group
 
     // The lane's own zone of the next item, and for lanes 0-3 the four guard
     // zones of that item (c0 - 2, c0 - 1, c0 + TJ, c0 + TJ + 1: neighbour
@@ -83,9 +91,60 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
     double nxt[NC] = {}, gnxt[NC] = {};
     if (blockIdx.x < items) fetch(blockIdx.x, nxt, gnxt);
 
+    // extended-array mode (per-block compatibility): zone index i in
+    // [-2, n+2) of the caller's primitive array
+    auto load_pe = [&] (int i, R* p) {
+        int e = i;
+        if (a.outflow_lower && i < 0) e = 0;
+        else if (!a.outflow_lower && a.outflow_upper && i >= n) e = n - 1;
+        for (int q = 0; q < NC; ++q) p[q] = R(a.pe[(size_t)q * (n + 4) + e + 2]);
+    };
+    // The left face of item `it` (zones c0 - 2 .. c0 + 1), with the operands
+    // and arithmetic of every other face.  A tile of TJ zones has TJ + 1
+    // faces; each lane computes the right face of its zone, and the left
+    // faces of a group of TJ items come from one pass of all lanes, so no
+    // item needs a second, nearly idle flux pass.  Recovery errors in these
+    // zones are reported by the items that own them.
+    auto left_face = [&] (long it, R* f) {
+        const int b = (int)(it / tiles), c0 = (int)(it % tiles) * TJ;
+        R pz[4][NC];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = c0 - 2 + r;
+            if (a.pe) {
+                load_pe(i, pz[r]);
+            } else {
+                const double* src = i < 0 ? guard_zone(b, i) : a.u + (size_t)b * NC * n + i;
+                double uu[NC];
+#pragma unroll
+                for (int q = 0; q < NC; ++q) uu[q] = src[(size_t)q * n];
+                (void)Euler<R>::recover(uu, prm, pz[r]);
+            }
+        }
+        R pl[NC], pr[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            pl[q] = pz[1][q] + plm_half(pz[0][q], pz[1][q], pz[2][q], theta);
+            pr[q] = pz[2][q] - plm_half(pz[1][q], pz[2][q], pz[3][q], theta);
+        }
+        Euler<R>::template riemann<1>(pl, pr, prm, f);
+    };
+
     bool ok = true;
     double rate = 0.0;                  // fused CFL reduction
     for (long item = blockIdx.x; item < items; item += gridDim.x) {
+        // position of this item in its group of TJ consecutive items of this
+        // workgroup; the group's left faces are computed at its first item
+        const int gpos = (int)(((item - blockIdx.x) / gridDim.x) % TJ);
+        if (kGroupLeftFaces && gpos == 0) {
+            const long li = item + (long)t * gridDim.x;
+            if (li < items) {
+                R f[NC];
+                left_face(slot(li), f);
+#pragma unroll
+                for (int q = 0; q < NC; ++q) Fl[q][t] = f[q];
+            }
+        }
         const long it = slot(item);
         const int b = (int)(it / tiles);
         const int c0 = (int)(it % tiles) * TJ;
@@ -99,15 +158,6 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
 #pragma unroll
             for (int q = 0; q < NC; ++q) U0[q] = a.u0[(size_t)b * NC * n + (size_t)q * n + c0 + t];
         }
-        // extended-array mode (per-block compatibility): zone index i in
-        // [-2, n+2) of the caller's primitive array
-        auto load_pe = [&] (int i, R* p) {
-            int e = i;
-            if (a.outflow_lower && i < 0) e = 0;
-            else if (!a.outflow_lower && a.outflow_upper && i >= n) e = n - 1;
-            for (int q = 0; q < NC; ++q) p[q] = R(a.pe[(size_t)q * (n + 4) + e + 2]);
-        };
-
         R p[NC];
         if (a.pe) load_pe(c0 + t, p);
         else ok &= Euler<R>::recover(uz, prm, p);
@@ -119,17 +169,34 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
             for (int q = 0; q < NC; ++q) P[q][slot] = p[q];
         }
         __syncthreads();
-        for (int i = t; i < TJ + 2; i += TJ)
-            for (int q = 0; q < NC; ++q) G[q][i] = plm_half(P[q][i], P[q][i + 1], P[q][i + 2], theta);
-        __syncthreads();
-        for (int i = t; i < TJ + 1; i += TJ) {
+        {
+            // the right face of zone c0 + t (face t + 1), from the half slopes
+            // at zones c0 + t and c0 + t + 1
             R pl[NC], pr[NC], f[NC];
+#pragma unroll
             for (int q = 0; q < NC; ++q) {
-                pl[q] = P[q][i + 1] + G[q][i];
-                pr[q] = P[q][i + 2] - G[q][i + 1];
+                pl[q] = P[q][t + 2] + plm_half(P[q][t + 1], P[q][t + 2], P[q][t + 3], theta);
+                pr[q] = P[q][t + 3] - plm_half(P[q][t + 2], P[q][t + 3], P[q][t + 4], theta);
             }
             Euler<R>::template riemann<1>(pl, pr, prm, f);
-            for (int q = 0; q < NC; ++q) F[q][i] = f[q];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) F[q][t + 1] = f[q];
+            if (t == 0) {
+                if constexpr (kGroupLeftFaces) {
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) F[q][0] = Fl[q][gpos];
+                } else {
+                    // the left face from the LDS row (zones c0 - 2 .. c0 + 1)
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) {
+                        pl[q] = P[q][1] + plm_half(P[q][0], P[q][1], P[q][2], theta);
+                        pr[q] = P[q][2] - plm_half(P[q][1], P[q][2], P[q][3], theta);
+                    }
+                    Euler<R>::template riemann<1>(pl, pr, prm, f);
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) F[q][0] = f[q];
+                }
+            }
         }
         __syncthreads();
 
@@ -147,7 +214,9 @@ stage1d_kernel(EulerParams prm, Stage1DArgs a)
         }
         if (a.rate_slot) rate = fmax(rate, Euler<double>::max_speed(wv, prm) * idl);
         if (!ok) { report_error(a.err, 1, b, z); ok = true; }
-        __syncthreads();   // LDS is reused by the next item
+        // No barrier here: the next item writes P and Fl only after this
+        // item's flux barrier, which every read of them precedes, and F only
+        // after its own first barrier, which follows this item's update.
     }
     if (a.rate_slot) workgroup_atomic_max<TJ>(rate, a.rate_slot);
 }
