@@ -356,7 +356,7 @@ def bench_mhd(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1", file=sys.stderr)
+        print("bench.py: measures one GPU only (the library has no multi-GPU path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
