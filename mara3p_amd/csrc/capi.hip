@@ -369,6 +369,8 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
         return launch_status("m3p_mhd_advance_ct");
     }
     hipLaunchKernelGGL(mhd_faces_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)P, F1, E1, F2, E2, F3, E3);
+    hipLaunchKernelGGL(mhd_boundary_faces_kernel<R>, dim3((int)((3L * N * N + 255) / 256)), dim3(256), 0, s, a,
+                       (const R*)P, F1, E1, F2, E2, F3, E3);
     hipLaunchKernelGGL(mhd_update_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)F1, (const R*)F2, (const R*)F3, fe);
     return launch_status("m3p_mhd_advance_ct");
 }

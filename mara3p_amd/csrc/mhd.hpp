@@ -5,7 +5,8 @@
 //                          (primitive_array :214-224, face_to_cell
 //                          src/mesh_cartesian_3d.hpp:235-245) + CFL max speed
 //   2. mhd_faces_kernel    PLM face states, HLLE Godunov flux and face electric
-//                          field of the three faces below each cell
+//                          field of the three faces below each cell (and
+//                          mhd_boundary_faces_kernel for the upper planes)
 //                          (godunov_fluxes_plm :152-179,
 //                          mhd::riemann_hlle src/physics_mhd.hpp:358-390)
 //   3. mhd_update_kernel   U - div(F) dt / dl (divergence_difference
@@ -336,9 +337,24 @@ mhd_faces_kernel(MhdArgs a, const R* P, R* F1, R* E1, R* F2, R* E2, R* F3, R* E3
     mhd_one_face<R, 1>(a, P, F1, E1, i, j, k);
     mhd_one_face<R, 2>(a, P, F2, E2, i, j, k);
     mhd_one_face<R, 3>(a, P, F3, E3, i, j, k);
-    if (i == N - 1) mhd_one_face<R, 1>(a, P, F1, E1, N, j, k);
-    if (j == N - 1) mhd_one_face<R, 2>(a, P, F2, E2, i, N, k);
-    if (k == N - 1) mhd_one_face<R, 3>(a, P, F3, E3, i, j, N);
+}
+
+// The faces on the upper boundary planes (i = N, j = N, k = N), one per
+// thread: 3 N^2 faces.  In the per-cell pass the cells next to them would
+// run a fourth, nearly empty face pass (a quarter of all waves, for k = N).
+template<typename R>
+__global__ void __launch_bounds__(256)
+mhd_boundary_faces_kernel(MhdArgs a, const R* P, R* F1, R* E1, R* F2, R* E2, R* F3, R* E3)
+{
+    const int N = a.N;
+    const long NN = (long)N * N;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= 3 * NN) return;
+    const int axis = (int)(c / NN);
+    const int u = (int)(c % NN / N), v = (int)(c % N);
+    if (axis == 0)      mhd_one_face<R, 1>(a, P, F1, E1, N, u, v);
+    else if (axis == 1) mhd_one_face<R, 2>(a, P, F2, E2, u, N, v);
+    else                mhd_one_face<R, 3>(a, P, F3, E3, u, v, N);
 }
 
 // Column-tile form of the flux pass fused with the conserved update.  A
