@@ -11,10 +11,23 @@
  *
  * Every entry point returns an m3p status code.  Stream arguments are
  * hipStream_t passed as void*.  Stage entry points are asynchronous on that
- * stream; errors raised inside kernels (e.g. negative density) are latched in
- * a per-device slot and reported by m3p_check_errors, which the per-block
- * entry points call before returning (so they behave like the reference's
- * throwing functions).
+ * stream; errors raised inside their kernels (e.g. negative density) are
+ * latched in a slot of that stream and reported by m3p_check_errors(stream).
+ * The per-block entry points are blocking: each call latches its kernels'
+ * errors in a slot of its own and returns the status before returning (so
+ * they behave like the reference's throwing functions).
+ *
+ * Threads: any number of host threads may call the library at once.  The
+ * per-block calls are safe on any stream, the same one included (the
+ * reference calls updated_conserved concurrently, one block per thread-pool
+ * worker, src/parallel_computable.cpp:248-296).  The asynchronous calls keep
+ * their error latch and reduction scratch per stream, so concurrent callers
+ * use one stream each; calls on one stream are ordered by the stream.  A
+ * stream's state is created by its first call, which must come before any
+ * graph capture on it (m3p_graph_begin makes it).  Graph replays report
+ * kernel errors to the latch of the stream they were captured on.  The tile
+ * table (m3p_set_stage2d_tile, m3p_autotune_*) is read and written
+ * atomically.
  */
 #ifndef MARA3P_AMD_H
 #define MARA3P_AMD_H
@@ -35,14 +48,23 @@ extern "C" {
 #define M3P_KIND_MINIDISK  1
 #define M3P_KIND_AMR_EULER2D 2
 
-#define M3P_FP64   0   /* drop-in: reproduces the reference FP64 arithmetic */
-#define M3P_MIXED  1   /* FP32 reconstruction and fluxes, FP64 conserved state */
+/* Precision modes.  M3P_FP64 is the drop-in: FP64 in the reference's
+ * association order.  Its parity bar per scheme: Euler (1D, 2D, quad-tree,
+ * HLLE and HLLC), iso2d / locally isothermal, SRHD and the mhd_scheme_v2
+ * array functions equal the reference bit for bit; minidisk and MHD
+ * constrained transport agree within 1e-12 relative (minidisk: the device
+ * exp and the reciprocal forms of its sources differ from glibc and from the
+ * reference's divisions by an ulp or so; MHD: reciprocal forms in the wave
+ * speeds and the HLLE flux).  M3P_MIXED computes primitives, slopes and
+ * fluxes in FP32 and keeps the conserved state and flux differences in FP64. */
+#define M3P_FP64   0
+#define M3P_MIXED  1
 
 /* Library / error handling ------------------------------------------------ */
 int         m3p_version(void);
 const char* m3p_last_error(void);       /* text of the last failure on this thread */
-/* Synchronise `stream`, read and clear the device error latch.  *code is the
- * kernel status (M3P_OK if none), block/zone locate the first offender. */
+/* Synchronise `stream`, read and clear its error latch.  *code is the kernel
+ * status (M3P_OK if none), block/zone locate the first offender. */
 int         m3p_check_errors(void* stream, int* code, int* block, int* zone);
 
 /* Per-block entry points (compatibility path) ---------------------------- */
@@ -95,6 +117,18 @@ int m3p_minidisk_updated_conserved(const double* uc, const double* pe, double* o
 int m3p_locally_isothermal_updated_conserved(const double* uc, const double* pe, double* out, int n,
                                    double dt, double dl, const double* cs2_x, const double* cs2_y,
                                    double kinematic_viscosity, int precision, void* stream);
+
+/* locally_isothermal::updated_conserved over every block of a uniform mesh
+ * (the per-block update of updated_conserved_tree,
+ * src/module_locally_isothermal.cpp:218-238, with its guard fill fused as in
+ * m3p_euler2d_stage): nc = 3, nbr as m3p_euler2d_stage, face sound speeds
+ * squared per block cs2_x [nblk][(n+1)][n], cs2_y [nblk][n][(n+1)] (device),
+ * viscous stress when kinematic_viscosity != 0.  Asynchronous; the RK
+ * combine and dt_dev as m3p_euler2d_stage. */
+int m3p_locally_isothermal_stage(const double* u, const double* u0, double* out, const int* nbr,
+                                 int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
+                                 const double* cs2_x, const double* cs2_y, double kinematic_viscosity,
+                                 int precision, void* stream);
 
 /* Block helpers of the module API --------------------------------------- */
 /* recover_primitive_array, src/module_euler.cpp:84-89 (euler1d) and :240-245
@@ -279,8 +313,7 @@ int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, con
  * and dt_next (device double) receives the time step m3p_euler_cfl_dt /
  * m3p_euler_cfl_dt_blocks would compute from `out` with this cfl, bit for
  * bit.  dt_next is written after the stage has read dt_dev, so the two may be
- * the same buffer.  Shares the library's reduction scratch with the
- * m3p_euler_cfl_dt* calls (calls on different streams must not overlap). */
+ * the same buffer.  The reduction slots are those of `stream`. */
 int m3p_euler2d_stage_cfl(const double* u, const double* u0, double* out, const int* nbr,
                           int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
                           double plm_theta, double gamma_law_index, int hllc, double cfl, double* dt_next,

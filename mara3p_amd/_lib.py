@@ -94,6 +94,7 @@ SIGNATURES = {
     "m3p_euler1d_updated_conserved": (_I, [_P, _P, _P, _I, _D, _D, _D, _D, _I, _I, _I, _P]),
     "m3p_minidisk_updated_conserved": (_I, [_P, _P, _P, _I, _I, _I, _D, _D, ctypes.POINTER(MinidiskData), _I, _P]),
     "m3p_locally_isothermal_updated_conserved": (_I, [_P, _P, _P, _I, _D, _D, _P, _P, _D, _I, _P]),
+    "m3p_locally_isothermal_stage": (_I, [_P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _P, _P, _D, _I, _P]),
     "m3p_euler2d_stage": (_I, [_P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _D, _I, _I, _P]),
     "m3p_minidisk_stage": (_I, [_P, _P, _P, _P, _P, _I, _D, _D, _D, ctypes.POINTER(MinidiskData), _I, _P]),
     "m3p_euler1d_stage": (_I, [_P, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _I, _P]),

@@ -19,13 +19,16 @@ CSRC = os.path.join(PKG, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 REFERENCE = os.environ.get("MARA3P_REFERENCE", "/root/reference")
 
-HIP_SOURCES = ["capi.hip"]
+# capi.hip holds the entry points; the k_*.hip units compile the stage kernels
+# (one scheme family each), in parallel.
+HIP_SOURCES = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
 HIP_FLAGS = [
-    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
     "-ffp-contract=off", "-Wno-pass-failed",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
+OBJ_DIR = os.path.join(PKG, "_build")
 
 
 def _run(cmd, verbose):
@@ -39,11 +42,20 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build_hip(verbose=False):
+def build_hip(verbose=False, jobs=None):
+    """Compile every unit under csrc (in parallel) and link the library."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJ_DIR, exist_ok=True)
     target = os.path.join(PKG, "libmara3p_amd.so")
     tmp = target + ".tmp"
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([HIPCC] + HIP_FLAGS + srcs + ["-o", tmp], verbose)
+    objs = [os.path.join(OBJ_DIR, s[:-4] + ".o") for s in HIP_SOURCES]
+    jobs = jobs or min(len(HIP_SOURCES), max(1, min(os.cpu_count() or 1, 16)))
+
+    def compile_one(k):
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, HIP_SOURCES[k]), "-o", objs[k]], verbose)
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(compile_one, range(len(HIP_SOURCES))))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, target)
     return target
 

@@ -1,26 +1,25 @@
 // C ABI of the MI355X Godunov block update (declarations: include/mara3p_amd.h).
+// The stage kernels are compiled in the k_*.hip units (launch.hpp); this unit
+// holds the entry points, the library state and the small kernels.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
 #include <mutex>
 #include <string>
-#include "mara3p_amd.h"
+#include <unordered_map>
+#include <vector>
+#include "launch.hpp"
 #include "amr.hpp"
-#include "viscous2d.hpp"
 #include "elementwise.hpp"
-#include "mhd.hpp"
 #include "mhd_v2.hpp"
 #include "srhd.hpp"
-#include "stage1d.hpp"
-#include "stage2d.hpp"
 #include "two_body.hpp"
 
-using namespace m3p;
-
-namespace {
+namespace m3p {
 
 thread_local std::string g_last_error;
 
@@ -36,60 +35,188 @@ int hip_check(hipError_t e, const char* what)
     return fail(M3P_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Per-device library state: the kernel error latch and reduction scratch.
-struct DeviceState {
-    int* err = nullptr;          // int[4]
-    unsigned long long* red = nullptr;  // reduction scratch, kRedWords words
-    int cus = 0;                 // compute units (persistent-grid sizing)
-};
-
-constexpr int kMaxDevices = 64;
-constexpr int kRedWords = 4096;
-// reduction scratch: words [0, 2048) totals partials, then single slots
-constexpr int kSlotDivB = 3000;
-constexpr int kSlotCflRate = 3008;           // kRateSlots words
-DeviceState g_dev[kMaxDevices];
-std::mutex g_mutex;
-
-int device_state(DeviceState** out)
-{
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
-        return fail(M3P_ERR_HIP, "m3p: no current HIP device");
-    std::lock_guard<std::mutex> lock(g_mutex);
-    DeviceState& s = g_dev[dev];
-    if (!s.err) {
-        if (int st = hip_check(hipMalloc(&s.err, 4 * sizeof(int)), "hipMalloc(error latch)")) return st;
-        if (int st = hip_check(hipMemset(s.err, 0, 4 * sizeof(int)), "hipMemset(error latch)")) return st;
-        if (int st = hip_check(hipMalloc(&s.red, kRedWords * sizeof(unsigned long long)), "hipMalloc(reduction)")) return st;
-        if (int st = hip_check(hipDeviceGetAttribute(&s.cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute")) return st;
-    }
-    *out = &s;
-    return M3P_OK;
-}
-
 int launch_status(const char* what)
 {
     return hip_check(hipGetLastError(), what);
 }
 
+} // namespace m3p
+
+using namespace m3p;
+
+namespace {
+
 // ---------------------------------------------------------------------------
-// 2D launch dispatch.  A tile is TJ columns x H rows; the compiled variants
-// are listed below.  The variant used for each (scheme, precision) is a
-// per-process setting chosen by m3p_set_stage2d_tile or m3p_autotune_*.
-struct TileShape { int tj, h; };
-constexpr TileShape kTileVariants[] = { {256, 128}, {256, 64}, {256, 32}, {256, 16}, {128, 64}, {128, 32}, {64, 64}, {64, 32}, {64, 16}, {32, 32}, {16, 16} };
-constexpr int kNumVariants = sizeof(kTileVariants) / sizeof(kTileVariants[0]);
+// Library state.  Kernels report errors (negative density, ...) into a latch
+// of four ints, and the reductions keep partial results in a scratch buffer.
+// Both are per stream: an asynchronous call uses the latch and scratch of its
+// stream, so callers on different streams never see each other's errors and
+// never share reduction slots, and calls on one stream are ordered by it.
+// The blocking per-block calls instead lease a latch of their own from a
+// per-device pool for the duration of the call, so they are safe from any
+// number of host threads on any stream, the null stream included.
+constexpr int kRedWords = 4096;
+// reduction scratch: words [0, 2048) totals partials, then single slots
+constexpr int kSlotDivB = 3000;
+constexpr int kSlotCflRate = 3008;           // kRateSlots words
+
+struct StreamState {
+    int* err = nullptr;                  // int[4]: code, block, zone
+    unsigned long long* red = nullptr;   // kRedWords words
+    int cus = 0;                         // compute units of the device
+};
+
+struct DeviceState {
+    int cus = 0;
+    std::unordered_map<hipStream_t, StreamState> streams;
+    std::vector<int*> free_latches;
+};
+
+constexpr int kMaxDevices = 64;
+DeviceState g_dev[kMaxDevices];
+std::mutex g_mutex;                          // guards g_dev
+
+int current_device(int* dev)
+{
+    if (hipGetDevice(dev) != hipSuccess || *dev < 0 || *dev >= kMaxDevices)
+        return fail(M3P_ERR_HIP, "m3p: no current HIP device");
+    return M3P_OK;
+}
+
+// A zeroed latch on the current device (synchronous copy, no memset node).
+int new_latch(int** out)
+{
+    const int zero[4] = { 0, 0, 0, 0 };
+    if (int st = hip_check(hipMalloc(out, sizeof(zero)), "hipMalloc(error latch)")) return st;
+    return hip_check(hipMemcpy(*out, zero, sizeof(zero), hipMemcpyHostToDevice), "hipMemcpy(error latch)");
+}
+
+int device_cus(DeviceState& d, int dev)
+{
+    if (d.cus == 0)
+        if (int st = hip_check(hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, dev),
+                               "hipDeviceGetAttribute")) return st;
+    return M3P_OK;
+}
+
+// The state of (current device, stream), created on first use.  Creating it
+// allocates, which a stream under graph capture must not do: the first call
+// on a stream has to come before its capture begins (m3p_graph_begin does it).
+int stream_state(void* stream, StreamState** out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    int dev;
+    if (int st = current_device(&dev)) return st;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState& d = g_dev[dev];
+    auto it = d.streams.find(s);
+    if (it != d.streams.end()) { *out = &it->second; return M3P_OK; }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p: the first call on a stream must come before its graph capture "
+                                          "(begin captures with m3p_graph_begin)");
+    if (int st = device_cus(d, dev)) return st;
+    StreamState ss;
+    ss.cus = d.cus;
+    if (int st = new_latch(&ss.err)) return st;
+    if (int st = hip_check(hipMalloc(&ss.red, kRedWords * sizeof(unsigned long long)), "hipMalloc(reduction)")) {
+        (void)hipFree(ss.err);
+        return st;
+    }
+    *out = &(d.streams[s] = ss);
+    return M3P_OK;
+}
+
+// The latch of one blocking call: leased from the device's pool, read after
+// the call's kernels have finished, cleared if set, and given back.
+struct Lease {
+    int* latch = nullptr;
+    int cus = 0;
+    int dev = -1;
+    bool pending = false;      // kernels may still write the latch
+    hipStream_t stream = nullptr;
+
+    int acquire(void* s)
+    {
+        stream = (hipStream_t)s;
+        if (int st = current_device(&dev)) return st;
+        {
+            std::lock_guard<std::mutex> lock(g_mutex);
+            DeviceState& d = g_dev[dev];
+            if (int st = device_cus(d, dev)) return st;
+            cus = d.cus;
+            if (!d.free_latches.empty()) {
+                latch = d.free_latches.back();
+                d.free_latches.pop_back();
+            }
+        }
+        if (!latch)
+            if (int st = new_latch(&latch)) { latch = nullptr; return st; }
+        pending = true;
+        return M3P_OK;
+    }
+    // Wait for the call's kernels; the latch's code, block and zone.
+    int wait(int* code, int* block, int* zone)
+    {
+        int h[4] = { 0, 0, 0, 0 };
+        if (int st = hip_check(hipMemcpyAsync(h, latch, sizeof(h), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(error latch)")) return st;
+        if (int st = hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize")) return st;
+        pending = false;
+        if (h[0] != 0) {
+            const int zero[4] = { 0, 0, 0, 0 };
+            if (int st = hip_check(hipMemcpy(latch, zero, sizeof(zero), hipMemcpyHostToDevice), "hipMemcpy(error latch)")) return st;
+        }
+        if (code) *code = h[0];
+        if (block) *block = h[1];
+        if (zone) *zone = h[2];
+        return M3P_OK;
+    }
+    // wait(), with a latched negative density reported as the reference throws it.
+    int finish()
+    {
+        int code = 0, block = 0, zone = 0;
+        if (int st = wait(&code, &block, &zone)) return st;
+        if (code == M3P_ERR_NEGATIVE_DENSITY) return fail(code, "euler::recover_primitive (negative density)");
+        if (code) return fail(code, "m3p: kernel error " + std::to_string(code));
+        return M3P_OK;
+    }
+    ~Lease()
+    {
+        if (!latch) return;
+        if (pending) {
+            // a launch failed after an earlier one of the call was queued:
+            // wait for it and clear what it may have written
+            if (wait(nullptr, nullptr, nullptr) != M3P_OK) return;     // unusable: not returned to the pool
+            const int zero[4] = { 0, 0, 0, 0 };
+            if (hipMemcpy(latch, zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess) return;
+        }
+        std::lock_guard<std::mutex> lock(g_mutex);
+        g_dev[dev].free_latches.push_back(latch);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// 2D launch dispatch.  The variant used for each (scheme kind, precision) is
+// a per-process setting (an index into kTileVariants) chosen by
+// m3p_set_stage2d_tile or m3p_autotune_*; it is read and written atomically.
 constexpr int kNumKinds = 3;
+constexpr int tile_index(int tj, int h)
+{
+    for (int i = 0; i < kNumVariants; ++i)
+        if (kTileVariants[i].tj == tj && kTileVariants[i].h == h) return i;
+    return -1;
+}
 // [kind][precision]: defaults from sweeps of the benchmark meshes; the tree
 // kind runs 64^2 leaves, where 64 x 16 tiles give the most workgroups.
-TileShape g_tile[kNumKinds][2] = { { {256, 16}, {256, 16} }, { {256, 16}, {256, 16} }, { {64, 16}, {64, 16} } };
+std::atomic<int> g_tile[kNumKinds][2] = { { tile_index(256, 16), tile_index(256, 16) },
+                                          { tile_index(256, 16), tile_index(256, 16) },
+                                          { tile_index(64, 16), tile_index(64, 16) } };
 
 bool tile_fits(TileShape v, int n) { return n % v.tj == 0 && n % v.h == 0 && v.h <= n; }
 
 TileShape pick_tile(int kind, int precision, int n)
 {
-    TileShape want = g_tile[kind][precision];
+    TileShape want = kTileVariants[g_tile[kind][precision].load(std::memory_order_relaxed)];
     if (tile_fits(want, n)) return want;
     // widest fitting tile, then the fewest rows (most workgroups)
     TileShape best{0, 0};
@@ -98,47 +225,13 @@ TileShape pick_tile(int kind, int precision, int n)
     return best;
 }
 
-template<class S, class Ld, bool FIRST>
-int launch_stage2d_variant(TileShape ts, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
-{
-    // The mixed kernels keep the conserved state in registers between its load
-    // and its update; the FP64 kernels (register-bound) re-load it.
-    constexpr bool K = !is_fp64<typename S::Real>::value;
-    const int n = a.n;
-    auto go = [&] (auto kern) {
-        const int grid = a.nblk * (n / ts.h) * (n / ts.tj);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(ts.tj), 0, s, prm, ld, a);
-        return launch_status("stage2d_kernel");
-    };
-    if (ts.tj == 256 && ts.h == 128) return go(stage2d_kernel<S, Ld, 128, 256, K, FIRST>);
-    if (ts.tj == 256 && ts.h == 64) return go(stage2d_kernel<S, Ld, 64, 256, K, FIRST>);
-    if (ts.tj == 256 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 256, K, FIRST>);
-    if (ts.tj == 256 && ts.h == 16) return go(stage2d_kernel<S, Ld, 16, 256, K, FIRST>);
-    if (ts.tj == 128 && ts.h == 64) return go(stage2d_kernel<S, Ld, 64, 128, K, FIRST>);
-    if (ts.tj == 128 && ts.h == 32) return go(stage2d_kernel<S, Ld, 32, 128, K, FIRST>);
-    if (ts.tj == 64 && ts.h == 64)  return go(stage2d_kernel<S, Ld, 64, 64, K, FIRST>);
-    if (ts.tj == 64 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 64, K, FIRST>);
-    if (ts.tj == 64 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 64, K, FIRST>);
-    if (ts.tj == 32 && ts.h == 32)  return go(stage2d_kernel<S, Ld, 32, 32, K, FIRST>);
-    if (ts.tj == 16 && ts.h == 16)  return go(stage2d_kernel<S, Ld, 16, 16, K, FIRST>);
-    return fail(M3P_ERR_BAD_SHAPE, "m3p: 2D block size must be 16, 32 or a multiple of 64 (got " + std::to_string(n) + ")");
-}
-
-template<class S, class Ld>
-int launch_stage2d_shape(TileShape ts, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
-{
-    // A stage without a base state has its own kernels only where the scheme
-    // fetches more rows ahead there (and the loader pipelines its rows).
-    if constexpr (S::kPrefetchRows > 1 && Ld::kPrefetch)
-        if (!a.u0) return launch_stage2d_variant<S, Ld, true>(ts, prm, ld, a, s);
-    return launch_stage2d_variant<S, Ld, false>(ts, prm, ld, a, s);
-}
-
 template<class S, class Ld>
 int launch_stage2d(int kind, const typename S::Params& prm, const Ld& ld, const StageArgs& a, hipStream_t s)
 {
     const int prec = is_fp64<typename S::Real>::value ? M3P_FP64 : M3P_MIXED;
-    return launch_stage2d_shape<S, Ld>(pick_tile(kind, prec, a.n), prm, ld, a, s);
+    const TileShape ts = pick_tile(kind, prec, a.n);
+    if (ts.tj == 0) return fail(M3P_ERR_BAD_SHAPE, "m3p: 2D block size must be 16, 32 or a multiple of 64 (got " + std::to_string(a.n) + ")");
+    return launch_stage2d_shape<S, Ld>(ts, prm, ld, a, s);
 }
 
 // Time every tile variant that fits on the caller's arrays and keep the fastest.
@@ -149,66 +242,33 @@ int autotune_stage2d(int kind, const typename S::Params& prm, const Ld& ld, cons
     const int prec = is_fp64<typename S::Real>::value ? M3P_FP64 : M3P_MIXED;
     hipEvent_t e0, e1;
     if (int st = hip_check(hipEventCreate(&e0), "hipEventCreate")) return st;
-    if (int st = hip_check(hipEventCreate(&e1), "hipEventCreate")) return st;
-    TileShape best{0, 0};
+    if (int st = hip_check(hipEventCreate(&e1), "hipEventCreate")) { (void)hipEventDestroy(e0); return st; }
+    int best = -1;
     float bms = 1e30f;
-    for (auto v : kTileVariants) {
+    int st = M3P_OK;
+    for (int i = 0; i < kNumVariants && !st; ++i) {
+        const TileShape v = kTileVariants[i];
         if (!tile_fits(v, a.n)) continue;
-        if (int st = launch_stage2d_shape<S, Ld>(v, prm, ld, a, s)) return st;       // warm-up
+        if ((st = launch_stage2d_shape<S, Ld>(v, prm, ld, a, s))) break;       // warm-up
         (void)hipEventRecord(e0, s);
-        for (int r = 0; r < reps; ++r)
-            if (int st = launch_stage2d_shape<S, Ld>(v, prm, ld, a, s)) return st;
+        for (int r = 0; r < reps && !st; ++r) st = launch_stage2d_shape<S, Ld>(v, prm, ld, a, s);
+        if (st) break;
         (void)hipEventRecord(e1, s);
-        if (int st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize")) return st;
+        if ((st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize"))) break;
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         ms /= reps;
-        if (ms < bms) { bms = ms; best = v; }
+        if (ms < bms) { bms = ms; best = i; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (best.tj == 0) return fail(M3P_ERR_BAD_SHAPE, "m3p: no tile variant fits block size " + std::to_string(a.n));
-    g_tile[kind][prec] = best;
-    if (best_tj) *best_tj = best.tj;
-    if (best_h) *best_h = best.h;
+    if (st) return st;
+    if (best < 0) return fail(M3P_ERR_BAD_SHAPE, "m3p: no tile variant fits block size " + std::to_string(a.n));
+    g_tile[kind][prec].store(best, std::memory_order_relaxed);
+    if (best_tj) *best_tj = kTileVariants[best].tj;
+    if (best_h) *best_h = kTileVariants[best].h;
     if (best_ms) *best_ms = bms;
     return M3P_OK;
-}
-
-template<typename R>
-int launch_stage1d(const EulerParams& prm, const Stage1DArgs& a, hipStream_t s, int cus)
-{
-    const int n = a.n;
-    // persistent grid: 16 (FP64) or 32 (mixed) waves per CU, somewhat above
-    // what is resident at once (about 160 and 100 VGPRs: 12 and 20 waves);
-    // the excess workgroups start as others finish.  A grid of exactly the
-    // resident workgroups (the runtime's occupancy query) measured 1.5 %
-    // faster in FP64 and 3 % slower in the mixed second stage.
-    const int waves_per_cu = is_fp64<R>::value ? 16 : 32;
-    auto go = [&] (auto kern, int TJ) {
-        const long items = (long)a.nblk * (n / TJ);
-        const int wgs_per_cu = std::max(1, waves_per_cu / std::max(1, TJ / 64));
-        const int grid = (int)std::min<long>(items, (long)std::max(cus, 1) * wgs_per_cu);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(TJ), 0, s, prm, a);
-        return launch_status("stage1d_kernel");
-    };
-    switch (n) {
-        case 16:  return go(stage1d_kernel<R, 16>, 16);
-        case 32:  return go(stage1d_kernel<R, 32>, 32);
-        case 64:  return go(stage1d_kernel<R, 64>, 64);
-        case 128: return go(stage1d_kernel<R, 128>, 128);
-        default:
-            if (n >= 256 && n % 256 == 0) return go(stage1d_kernel<R, 256>, 256);
-    }
-    return fail(M3P_ERR_BAD_SHAPE, "m3p: 1D block size must be 16, 32, 64, 128 or a multiple of 256 (got " + std::to_string(n) + ")");
-}
-
-template<class S, class Ld>
-int launch_visc(const typename S::Params& prm, const Ld& ld, const StageArgs& a, double nu, hipStream_t s)
-{
-    const long zones = (long)a.nblk * a.n * a.n;
-    hipLaunchKernelGGL((iso_visc_stage_kernel<S, Ld>), dim3((zones + 255) / 256), dim3(256), 0, s, prm, ld, a, nu);
-    return launch_status("iso_visc_stage_kernel");
 }
 
 MinidiskParams minidisk_params(const m3p_minidisk_data* d, double time)
@@ -243,14 +303,7 @@ int check_minidisk_data(const m3p_minidisk_data* d)
     return M3P_OK;
 }
 
-// ---------------------------------------------------------------------------
 // Reductions
-__global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long v)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 // CFL rate: max over zones of (max(|v1|, |v2|) + c_s) / dl_b, with the
 // reciprocal 1 / dl_b of the zone's block (dl_blk) or of the mesh (dl).  The
 // stage kernels with a fused reduction compute the same rate of the values
@@ -319,62 +372,6 @@ totals_final_kernel(const double* partial, int nparts, double* totals)
     if (threadIdx.x == 0) totals[q] = s;
 }
 
-// MHD constrained transport helpers
-long mhd_workspace_bytes(int N, int precision)
-{
-    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
-    const long rs = precision == M3P_FP64 ? 8 : 4;
-    return (8 * N3 + 3 * 7 * nf) * rs + 1024;    // + alignment, the speed slots and dt
-}
-
-template<typename R>
-int mhd_advance(MhdArgs a, char* work, hipStream_t s)
-{
-    const int N = a.N;
-    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
-    R* P = reinterpret_cast<R*>(work);
-    R* F1 = P + 8 * N3;
-    R* F2 = F1 + 5 * nf;
-    R* F3 = F2 + 5 * nf;
-    R* E1 = F3 + 5 * nf;
-    R* E2 = E1 + 2 * nf;
-    R* E3 = E2 + 2 * nf;
-    char* tail = reinterpret_cast<char*>(E3 + 2 * nf);
-    tail += (256 - (reinterpret_cast<uintptr_t>(tail) & 255)) & 255;
-    a.vmax = reinterpret_cast<unsigned long long*>(tail);
-    if (!a.dt) a.dt = reinterpret_cast<double*>(tail + 8 * kRateSlots);
-    const int gc = (int)((N3 + 255) / 256);
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, a.vmax, kRateSlots, 0ull);
-    hipLaunchKernelGGL(mhd_prim_kernel<R>, dim3(std::min(gc, 4096)), dim3(256), 0, s, a, P);
-    hipLaunchKernelGGL(mhd_dt_kernel, dim3(1), dim3(kRateSlots), 0, s, a);
-    FaceE<R> fe{ E1, E2, E3, N, nf };
-    if constexpr (!is_fp64<R>::value) if (N % 8 == 0 && N >= 8) {
-        // column tiles: fluxes and the conserved update in one pass, then the
-        // faces (the FP64 tile needs ~255 registers and runs slower than the
-        // separate passes, so FP64 keeps those)
-        // 16-plane segments: 4096 workgroups at 256^3, four full rounds of
-        // the 1024 resident at four waves per SIMD (32-plane segments run
-        // as fast; 16 x 16 column tiles 10 % slower)
-        const int TI = N >= 64 ? 16 : N;
-        const int segs = N / TI;
-        if (N % 32 == 0) {
-            const int grid = (N / 8) * (N / 32) * segs;
-            hipLaunchKernelGGL((mhd_tile_kernel<R, 8, 32>), dim3(grid), dim3(256), 0, s, a, (const R*)P, E1, E2, E3, TI);
-        } else {
-            const int grid = (N / 8) * (N / 8) * segs;
-            hipLaunchKernelGGL((mhd_tile_kernel<R, 8, 8>), dim3(grid), dim3(64), 0, s, a, (const R*)P, E1, E2, E3, TI);
-        }
-        if (int st = launch_status("mhd_tile_kernel")) return st;
-        hipLaunchKernelGGL(mhd_face_b_kernel<R>, dim3(gc), dim3(256), 0, s, a, fe);
-        return launch_status("m3p_mhd_advance_ct");
-    }
-    hipLaunchKernelGGL(mhd_faces_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)P, F1, E1, F2, E2, F3, E3);
-    hipLaunchKernelGGL(mhd_boundary_faces_kernel<R>, dim3((int)((3L * N * N + 255) / 256)), dim3(256), 0, s, a,
-                       (const R*)P, F1, E1, F2, E2, F3, E3);
-    hipLaunchKernelGGL(mhd_update_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)F1, (const R*)F2, (const R*)F3, fe);
-    return launch_status("m3p_mhd_advance_ct");
-}
-
 
 } // namespace
 
@@ -387,27 +384,19 @@ const char* m3p_last_error(void) { return g_last_error.c_str(); }
 
 int m3p_check_errors(void* stream, int* code, int* block, int* zone)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     hipStream_t s = (hipStream_t)stream;
     int h[4] = { 0, 0, 0, 0 };
+    if (int st = hip_check(hipMemcpyAsync(h, ds->err, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(error latch)")) return st;
     if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
-    if (int st = hip_check(hipMemcpy(h, ds->err, sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy(error latch)")) return st;
-    if (h[0] != 0)
-        if (int st = hip_check(hipMemset(ds->err, 0, sizeof(h)), "hipMemset(error latch)")) return st;
+    if (h[0] != 0) {
+        const int zero[4] = { 0, 0, 0, 0 };
+        if (int st = hip_check(hipMemcpy(ds->err, zero, sizeof(zero), hipMemcpyHostToDevice), "hipMemcpy(error latch)")) return st;
+    }
     if (code) *code = h[0];
     if (block) *block = h[1];
     if (zone) *zone = h[2];
-    return M3P_OK;
-}
-
-static int finish_blocking(void* stream)
-{
-    int code = 0, block = 0, zone = 0;
-    if (int st = m3p_check_errors(stream, &code, &block, &zone)) return st;
-    if (code == M3P_ERR_NEGATIVE_DENSITY)
-        return fail(code, "euler::recover_primitive (negative density)");
-    if (code) return fail(code, "m3p: kernel error " + std::to_string(code));
     return M3P_OK;
 }
 
@@ -415,12 +404,12 @@ int m3p_euler2d_updated_conserved(const double* uc, const double* pe, double* ou
                                   double dt, double dl, double plm_theta, double gamma_law_index,
                                   int hllc, int precision, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    Lease lease;
+    if (int st = lease.acquire(stream)) return st;
     if (!uc || !pe || !out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: NULL array");
     EulerParams prm{ gamma_law_index, plm_theta, hllc };
     StageArgs a{};
-    a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = ds->err;
+    a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = lease.latch;
     hipStream_t s = (hipStream_t)stream;
     int st;
     if (precision == M3P_FP64)
@@ -428,39 +417,39 @@ int m3p_euler2d_updated_conserved(const double* uc, const double* pe, double* ou
     else
         st = launch_stage2d<EulerScheme<float>>(M3P_KIND_EULER2D, prm, ExtendedLoader<EulerScheme<float>>{uc, pe, n}, a, s);
     if (st) return st;
-    return finish_blocking(stream);
+    return lease.finish();
 }
 
 int m3p_euler1d_updated_conserved(const double* uc, const double* pe, double* out, int n,
                                   double dt, double dl, double plm_theta, double gamma_law_index,
                                   int outflow_lower, int outflow_upper, int precision, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    Lease lease;
+    if (int st = lease.acquire(stream)) return st;
     if (!uc || !pe || !out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: NULL array");
     EulerParams prm{ gamma_law_index, plm_theta, 0 };
     Stage1DArgs a{};
-    a.u = uc; a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = ds->err;
+    a.u = uc; a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = lease.latch;
     a.pe = pe; a.outflow_lower = outflow_lower; a.outflow_upper = outflow_upper;
     hipStream_t s = (hipStream_t)stream;
-    int st = precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s, ds->cus) : launch_stage1d<float>(prm, a, s, ds->cus);
+    int st = precision == M3P_FP64 ? launch_stage1d<double>(prm, a, s, lease.cus) : launch_stage1d<float>(prm, a, s, lease.cus);
     if (st) return st;
-    return finish_blocking(stream);
+    return lease.finish();
 }
 
 int m3p_minidisk_updated_conserved(const double* uc, const double* pe, double* out,
                                    int level, int bi, int bj, double time, double dt,
                                    const m3p_minidisk_data* data, int precision, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    Lease lease;
+    if (int st = lease.acquire(stream)) return st;
     if (int st = check_minidisk_data(data)) return st;
     if (!uc || !pe || !out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: NULL array");
     const int n = data->block_size;
     MinidiskParams prm = minidisk_params(data, time);
     prm.level = level;
     StageArgs a{};
-    a.out = out; a.dt = dt; a.n = n; a.nblk = 1; a.err = ds->err;
+    a.out = out; a.dt = dt; a.n = n; a.nblk = 1; a.err = lease.latch;
     a.bi0 = bi; a.bj0 = bj;
     // cell_size, problems/minidisk_scheme.cpp:78-81
     a.dl = 2.0 * data->domain_radius / double(data->block_size) / double(1 << level);
@@ -476,7 +465,7 @@ int m3p_minidisk_updated_conserved(const double* uc, const double* pe, double* o
     else
         st = launch_stage2d<MinidiskScheme<float>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<MinidiskScheme<float>>{uc, pe, n}, a, s);
     if (st) return st;
-    return finish_blocking(stream);
+    return lease.finish();
 }
 
 static int euler2d_stage(const double* u, const double* u0, double* out, const int* nbr,
@@ -484,8 +473,8 @@ static int euler2d_stage(const double* u, const double* u0, double* out, const i
                          double plm_theta, double gamma_law_index, int hllc, int precision, void* stream,
                          unsigned long long* rate_slot)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !out || !nbr || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad euler2d stage arguments");
     if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
     EulerParams prm{ gamma_law_index, plm_theta, hllc };
@@ -510,8 +499,8 @@ int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int
                        int nblk, double time, double dt, double rk_b,
                        const m3p_minidisk_data* data, int precision, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (int st = check_minidisk_data(data)) return st;
     if (!u || !out || !nbr || !bcoord || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad minidisk stage arguments");
     if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
@@ -536,8 +525,8 @@ int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int*
                       int nblk, int n, double dt, double dl, double rk_b,
                       double plm_theta, double gamma_law_index, int precision, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !out || !nbr || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad euler1d stage arguments");
     if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
     EulerParams prm{ gamma_law_index, plm_theta, 0 };
@@ -551,8 +540,8 @@ int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int*
 int m3p_euler_cfl_dt(const double* u, int nblk, long zones_per_block, double gamma_law_index,
                      double cfl, double dl, double* dt_out, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !dt_out || nblk <= 0 || zones_per_block <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad cfl arguments");
     hipStream_t s = (hipStream_t)stream;
     EulerParams prm{ gamma_law_index, 0.0, 0 };
@@ -568,8 +557,8 @@ int m3p_euler_cfl_dt(const double* u, int nblk, long zones_per_block, double gam
 
 int m3p_conserved_totals(const double* u, int nblk, int nc, long zones_per_block, double* totals, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !totals || nblk <= 0 || nc <= 0 || nc > 8 || zones_per_block <= 0)
         return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad totals arguments");
     hipStream_t s = (hipStream_t)stream;
@@ -582,19 +571,19 @@ int m3p_conserved_totals(const double* u, int nblk, int nc, long zones_per_block
 
 int m3p_euler_recover_primitive_array(const double* u, double* p, long nzones, double gamma_law_index, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    Lease lease;
+    if (int st = lease.acquire(stream)) return st;
     if (!u || !p || nzones <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad recover arguments");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(euler_recover_array_kernel, dim3((nzones + 255) / 256), dim3(256), 0, s, u, p, nzones, gamma_law_index, ds->err);
+    hipLaunchKernelGGL(euler_recover_array_kernel, dim3((nzones + 255) / 256), dim3(256), 0, s, u, p, nzones, gamma_law_index, lease.latch);
     if (int st = launch_status("euler_recover_array_kernel")) return st;
-    return finish_blocking(stream);
+    return lease.finish();
 }
 
 int m3p_iso2d_recover_primitive_array(const double* u, double* p, long nzones, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !p || nzones <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad recover arguments");
     hipLaunchKernelGGL(iso2d_recover_array_kernel, dim3((nzones + 255) / 256), dim3(256), 0, (hipStream_t)stream, u, p, nzones);
     return launch_status("iso2d_recover_array_kernel");
@@ -602,8 +591,8 @@ int m3p_iso2d_recover_primitive_array(const double* u, double* p, long nzones, v
 
 int m3p_plm_gradient_2d(const double* p, double* g, int nc, int ni, int nj, int axis, double plm_theta, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!p || !g || nc <= 0 || axis < 0 || axis > 1 || (axis == 0 ? ni : nj) < 3 || ni <= 0 || nj <= 0)
         return fail(M3P_ERR_BAD_SHAPE, "m3p: bad gradient shape");
     long nout = (long)(axis == 0 ? ni - 2 : ni) * (axis == 1 ? nj - 2 : nj) * nc;
@@ -614,8 +603,8 @@ int m3p_plm_gradient_2d(const double* p, double* g, int nc, int ni, int nj, int 
 
 int m3p_weighted_sum(const double* s, const double* t, double* out, long n, double b, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!s || !t || !out || n <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad weighted_sum arguments");
     hipLaunchKernelGGL(weighted_sum_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, s, t, out, n, b);
     return launch_status("weighted_sum_kernel");
@@ -633,8 +622,8 @@ int m3p_mhd_advance_ct(const double* uc, const double* bf1, const double* bf2, c
                        double* uo, double* bo1, double* bo2, double* bo3, double* dt,
                        void* work, long work_bytes, int precision, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!uc || !bf1 || !bf2 || !bf3 || !uo || !bo1 || !bo2 || !bo3 || !work || N < 2)
         return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_advance_ct: bad arguments");
     if (u0 && (!b01 || !b02 || !b03)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_advance_ct: RK base needs all four arrays");
@@ -655,8 +644,8 @@ int m3p_mhd_advance_ct(const double* uc, const double* bf1, const double* bf2, c
 
 int m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, int N, double* out, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!bf1 || !bf2 || !bf3 || !out || N < 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_max_div_b: bad arguments");
     hipStream_t s = (hipStream_t)stream;
     const long N3 = (long)N * N * N;
@@ -670,14 +659,14 @@ int m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, i
 int m3p_srhd_recover_primitive_array(const double* u, double* p, long nzones, double gamma_law_index,
                                      double temperature_floor, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    Lease lease;
+    if (int st = lease.acquire(stream)) return st;
     if (!u || !p || nzones <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_srhd_recover_primitive_array: bad arguments");
     hipLaunchKernelGGL(srhd_recover_array_kernel, dim3((nzones + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                       u, p, nzones, gamma_law_index, temperature_floor, ds->err);
+                       u, p, nzones, gamma_law_index, temperature_floor, lease.latch);
     if (int st = launch_status("srhd_recover_array_kernel")) return st;
     int code = 0, block = 0, zone = 0;
-    if (int st = m3p_check_errors(stream, &code, &block, &zone)) return st;
+    if (int st = lease.wait(&code, &block, &zone)) return st;
     if (code == M3P_ERR_NO_CONVERGENCE) return fail(code, "srhd::recover_primitive (root finder not converging)");
     if (code == M3P_ERR_NEGATIVE_PRESSURE) return fail(code, "srhd::recover_primitive (negative pressure)");
     return code ? fail(code, "m3p: kernel error") : M3P_OK;
@@ -685,8 +674,8 @@ int m3p_srhd_recover_primitive_array(const double* u, double* p, long nzones, do
 
 int m3p_srhd_conserved_density_array(const double* p, double* u, long nzones, double gamma_law_index, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !p || nzones <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_srhd_conserved_density_array: bad arguments");
     hipLaunchKernelGGL(srhd_conserved_array_kernel, dim3((nzones + 255) / 256), dim3(256), 0, (hipStream_t)stream,
                        p, u, nzones, gamma_law_index);
@@ -696,15 +685,15 @@ int m3p_srhd_conserved_density_array(const double* p, double* u, long nzones, do
 int m3p_srhd_riemann_array(const double* pl, const double* pr, double* f, long nfaces, int axis,
                            double gamma_law_index, int mode, double face_speed, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    Lease lease;
+    if (int st = lease.acquire(stream)) return st;
     if (!pl || !pr || !f || nfaces <= 0 || axis < 1 || axis > 3 || mode < 0 || mode > 3)
         return fail(M3P_ERR_BAD_ARGUMENT, "m3p_srhd_riemann_array: bad arguments");
     hipLaunchKernelGGL(srhd_riemann_array_kernel, dim3((nfaces + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                       pl, pr, f, nfaces, axis, gamma_law_index, mode, face_speed, ds->err);
+                       pl, pr, f, nfaces, axis, gamma_law_index, mode, face_speed, lease.latch);
     if (int st = launch_status("srhd_riemann_array_kernel")) return st;
     int code = 0, block = 0, zone = 0;
-    if (int st = m3p_check_errors(stream, &code, &block, &zone)) return st;
+    if (int st = lease.wait(&code, &block, &zone)) return st;
     if (code) return fail(code, "srhd::riemann_solver (nan contact speed)");
     return M3P_OK;
 }
@@ -714,8 +703,8 @@ static int euler1d_stage_blocks(const double* u, const double* u0, double* out, 
                                 double plm_theta, double gamma_law_index, int precision, void* stream,
                                 unsigned long long* rate_slot)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !out || !nbr || !dl_blk || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_euler1d_stage_blocks: bad arguments");
     if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
     EulerParams prm{ gamma_law_index, plm_theta, 0 };
@@ -739,8 +728,8 @@ static int amr_euler2d_stage(const double* u, const double* u0, double* out, dou
                              double plm_theta, double gamma_law_index, int hllc, int precision, void* stream,
                              unsigned long long* rate_slot)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !out || !recipes || !dl_leaf || nleaf <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_amr_euler2d_stage: bad arguments");
     if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
     if (n < 16 || n % 2) return fail(M3P_ERR_BAD_SHAPE, "m3p_amr_euler2d_stage: leaf size must be an even number >= 16");
@@ -777,8 +766,8 @@ int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double
 // so dt_next may be the dt_dev the stage reads).
 static int cfl_stage(double cfl, double* dt_next, void* stream, const std::function<int(unsigned long long*)>& stage)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!dt_next || !(cfl > 0.0)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: a *_stage_cfl call needs cfl > 0 and dt_next");
     hipStream_t s = (hipStream_t)stream;
     unsigned long long* slot = ds->red + kSlotCflRate;
@@ -825,8 +814,8 @@ int m3p_amr_euler2d_stage_cfl(const double* u, const double* u0, double* out, do
 int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, const double* dl_blk,
                             double gamma_law_index, double cfl, double* dt_out, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !dl_blk || !dt_out || nblk <= 0 || zones_per_block <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad cfl arguments");
     hipStream_t s = (hipStream_t)stream;
     EulerParams prm{ gamma_law_index, 0.0, 0 };
@@ -842,8 +831,8 @@ int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, con
 // hipGraph capture of any sequence of asynchronous m3p calls on one stream.
 int m3p_graph_begin(void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;      // library state exists before capture starts
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;      // library state exists before capture starts
     return hip_check(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
 }
 
@@ -875,24 +864,25 @@ int m3p_graph_destroy(void* graph_exec)
 int m3p_set_stage2d_tile(int kind, int precision, int tj, int h)
 {
     if (kind < 0 || kind >= kNumKinds || precision < 0 || precision > 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad kind/precision");
-    for (auto v : kTileVariants)
-        if (v.tj == tj && v.h == h) { g_tile[kind][precision] = v; return M3P_OK; }
+    const int i = tile_index(tj, h);
+    if (i >= 0) { g_tile[kind][precision].store(i, std::memory_order_relaxed); return M3P_OK; }
     return fail(M3P_ERR_BAD_ARGUMENT, "m3p: tile " + std::to_string(tj) + "x" + std::to_string(h) + " is not a compiled variant");
 }
 
 int m3p_get_stage2d_tile(int kind, int precision, int* tj, int* h)
 {
     if (kind < 0 || kind >= kNumKinds || precision < 0 || precision > 1) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad kind/precision");
-    if (tj) *tj = g_tile[kind][precision].tj;
-    if (h) *h = g_tile[kind][precision].h;
+    const TileShape v = kTileVariants[g_tile[kind][precision].load(std::memory_order_relaxed)];
+    if (tj) *tj = v.tj;
+    if (h) *h = v.h;
     return M3P_OK;
 }
 
 int m3p_autotune_euler2d(const double* u, double* out, const int* nbr, int nblk, int n, double dl,
                          double gamma_law_index, int precision, int reps, int* tj, int* h, float* ms, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (!u || !out || !nbr || nblk <= 0 || u == out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad autotune arguments");
     EulerParams prm{ gamma_law_index, 1.5, 0 };
     StageArgs a{};
@@ -906,8 +896,8 @@ int m3p_autotune_euler2d(const double* u, double* out, const int* nbr, int nblk,
 int m3p_autotune_minidisk(const double* u, double* out, const int* nbr, const int* bcoord, int nblk,
                           const m3p_minidisk_data* data, int precision, int reps, int* tj, int* h, float* ms, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
     if (int st = check_minidisk_data(data)) return st;
     if (!u || !out || !nbr || !bcoord || nblk <= 0 || u == out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad autotune arguments");
     const int n = data->block_size;
@@ -925,13 +915,13 @@ int m3p_locally_isothermal_updated_conserved(const double* uc, const double* pe,
                                    double dt, double dl, const double* cs2_x, const double* cs2_y,
                                    double kinematic_viscosity, int precision, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    Lease lease;
+    if (int st = lease.acquire(stream)) return st;
     if (!uc || !pe || !out || !cs2_x || !cs2_y) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: NULL array");
     if (!(kinematic_viscosity >= 0.0)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: negative kinematic viscosity");
     IsoTableParams prm{ cs2_x, cs2_y, n };
     StageArgs a{};
-    a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = ds->err;
+    a.out = out; a.dt = dt; a.dl = dl; a.rk_b = 0.5; a.n = n; a.nblk = 1; a.err = lease.latch;
     hipStream_t s = (hipStream_t)stream;
     int st;
     if (kinematic_viscosity != 0.0)
@@ -943,7 +933,32 @@ int m3p_locally_isothermal_updated_conserved(const double* uc, const double* pe,
     else
         st = launch_stage2d<IsoTableScheme<float>>(M3P_KIND_MINIDISK, prm, ExtendedLoader<IsoTableScheme<float>>{uc, pe, n}, a, s);
     if (st) return st;
-    return finish_blocking(stream);
+    return lease.finish();
+}
+
+int m3p_locally_isothermal_stage(const double* u, const double* u0, double* out, const int* nbr,
+                                 int nblk, int n, double dt, const double* dt_dev, double dl, double rk_b,
+                                 const double* cs2_x, const double* cs2_y, double kinematic_viscosity,
+                                 int precision, void* stream)
+{
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
+    if (!u || !out || !nbr || !cs2_x || !cs2_y || nblk <= 0)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad locally isothermal stage arguments");
+    if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
+    if (!(kinematic_viscosity >= 0.0)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: negative kinematic viscosity");
+    IsoTableParams prm{ cs2_x, cs2_y, n };
+    StageArgs a{};
+    a.u0 = u0; a.out = out; a.dt_dev = dt_dev; a.dt = dt; a.dl = dl; a.rk_b = rk_b;
+    a.n = n; a.nblk = nblk; a.err = ds->err;
+    hipStream_t s = (hipStream_t)stream;
+    if (kinematic_viscosity != 0.0)
+        return precision == M3P_FP64
+             ? launch_visc<IsoTableScheme<double>>(prm, MeshLoader<IsoTableScheme<double>>{u, nbr, n}, a, kinematic_viscosity, s)
+             : launch_visc<IsoTableScheme<float>>(prm, MeshLoader<IsoTableScheme<float>>{u, nbr, n}, a, kinematic_viscosity, s);
+    if (precision == M3P_FP64)
+        return launch_stage2d<IsoTableScheme<double>>(M3P_KIND_MINIDISK, prm, MeshLoader<IsoTableScheme<double>>{u, nbr, n}, a, s);
+    return launch_stage2d<IsoTableScheme<float>>(M3P_KIND_MINIDISK, prm, MeshLoader<IsoTableScheme<float>>{u, nbr, n}, a, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -958,15 +973,15 @@ static bool v2_shape_ok(int a, int b, int c, int least) { return a >= least && b
 int m3p_mhd_v2_primitive_array(const double* uc, const double* bf1, const double* bf2, const double* bf3,
                                int n1, int n2, int n3, double* pc, void* stream)
 {
-    DeviceState* ds;
-    if (int st = device_state(&ds)) return st;
+    Lease lease;
+    if (int st = lease.acquire(stream)) return st;
     if (!uc || !bf1 || !bf2 || !bf3 || !pc) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_mhd_v2_primitive_array: NULL array");
     if (!v2_shape_ok(n1, n2, n3, 1)) return fail(M3P_ERR_BAD_SHAPE, "m3p_mhd_v2_primitive_array: empty block");
     hipLaunchKernelGGL(mhd_v2_prim_kernel, v2_grid((long)n1 * n2 * n3, 1), dim3(256), 0, (hipStream_t)stream,
-                       n1, n2, n3, uc, bf1, bf2, bf3, pc, ds->err);
+                       n1, n2, n3, uc, bf1, bf2, bf3, pc, lease.latch);
     if (int st = launch_status("mhd_v2_prim_kernel")) return st;
     int code = 0, block = 0, zone = 0;
-    if (int st = m3p_check_errors(stream, &code, &block, &zone)) return st;
+    if (int st = lease.wait(&code, &block, &zone)) return st;
     if (code == M3P_ERR_NEGATIVE_DENSITY) return fail(code, "mhd::recover_primitive (negative density)");
     if (code == M3P_ERR_NEGATIVE_PRESSURE) return fail(code, "mhd::recover_primitive (negative pressure)");
     return code ? fail(code, "m3p: kernel error " + std::to_string(code)) : M3P_OK;

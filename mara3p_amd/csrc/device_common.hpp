@@ -120,6 +120,14 @@ __device__ __forceinline__ void workgroup_atomic_max(double v, unsigned long lon
     }
 }
 
+// Fill n words with v (clears reduction slots; a kernel, not a memset, so a
+// captured graph replays it correctly).
+static __global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // Wave64 reductions through DPP/permute shuffles.
 __device__ __forceinline__ double wave_max(double v)
 {

@@ -1,0 +1,6 @@
+// Stage kernels of EulerScheme<float> (one unit per scheme and precision, compiled in parallel).
+#include "launch_impl.hpp"
+
+M3P_INSTANTIATE_STAGE2D(m3p::EulerScheme<float>, m3p::MeshLoader)
+M3P_INSTANTIATE_STAGE2D(m3p::EulerScheme<float>, m3p::ExtendedLoader)
+M3P_INSTANTIATE_STAGE2D(m3p::EulerScheme<float>, m3p::TreeLoader)

@@ -1,0 +1,66 @@
+// MHD constrained-transport stage passes (both precisions).
+#include <algorithm>
+#include "launch.hpp"
+
+namespace m3p {
+
+long mhd_workspace_bytes(int N, int precision)
+{
+    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
+    const long rs = precision == M3P_FP64 ? 8 : 4;
+    return (8 * N3 + 3 * 7 * nf) * rs + 1024;    // + alignment, the speed slots and dt
+}
+
+template<typename R>
+int mhd_advance(MhdArgs a, char* work, hipStream_t s)
+{
+    const int N = a.N;
+    const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
+    R* P = reinterpret_cast<R*>(work);
+    R* F1 = P + 8 * N3;
+    R* F2 = F1 + 5 * nf;
+    R* F3 = F2 + 5 * nf;
+    R* E1 = F3 + 5 * nf;
+    R* E2 = E1 + 2 * nf;
+    R* E3 = E2 + 2 * nf;
+    char* tail = reinterpret_cast<char*>(E3 + 2 * nf);
+    tail += (256 - (reinterpret_cast<uintptr_t>(tail) & 255)) & 255;
+    a.vmax = reinterpret_cast<unsigned long long*>(tail);
+    if (!a.dt) a.dt = reinterpret_cast<double*>(tail + 8 * kRateSlots);
+    const int gc = (int)((N3 + 255) / 256);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, a.vmax, kRateSlots, 0ull);
+    hipLaunchKernelGGL(mhd_prim_kernel<R>, dim3(std::min(gc, 4096)), dim3(256), 0, s, a, P);
+    hipLaunchKernelGGL(mhd_dt_kernel, dim3(1), dim3(kRateSlots), 0, s, a);
+    FaceE<R> fe{ E1, E2, E3, N, nf };
+    if constexpr (!is_fp64<R>::value) if (N % 8 == 0 && N >= 8) {
+        // column tiles: fluxes and the conserved update in one pass, then the
+        // faces (the FP64 tile needs ~255 registers and runs slower than the
+        // separate passes, so FP64 keeps those)
+        // 16-plane segments: 4096 workgroups at 256^3, four full rounds of
+        // the 1024 resident at four waves per SIMD (32-plane segments run
+        // as fast; 16 x 16 column tiles 10 % slower).  TI must divide N: an
+        // N with N % 16 == 8 takes 8-plane segments.
+        const int TI = N < 64 ? N : (N % 16 == 0 ? 16 : 8);
+        const int segs = N / TI;
+        if (N % 32 == 0) {
+            const int grid = (N / 8) * (N / 32) * segs;
+            hipLaunchKernelGGL((mhd_tile_kernel<R, 8, 32>), dim3(grid), dim3(256), 0, s, a, (const R*)P, E1, E2, E3, TI);
+        } else {
+            const int grid = (N / 8) * (N / 8) * segs;
+            hipLaunchKernelGGL((mhd_tile_kernel<R, 8, 8>), dim3(grid), dim3(64), 0, s, a, (const R*)P, E1, E2, E3, TI);
+        }
+        if (int st = launch_status("mhd_tile_kernel")) return st;
+        hipLaunchKernelGGL(mhd_face_b_kernel<R>, dim3(gc), dim3(256), 0, s, a, fe);
+        return launch_status("m3p_mhd_advance_ct");
+    }
+    hipLaunchKernelGGL(mhd_faces_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)P, F1, E1, F2, E2, F3, E3);
+    hipLaunchKernelGGL(mhd_boundary_faces_kernel<R>, dim3((int)((3L * N * N + 255) / 256)), dim3(256), 0, s, a,
+                       (const R*)P, F1, E1, F2, E2, F3, E3);
+    hipLaunchKernelGGL(mhd_update_kernel<R>, dim3(gc), dim3(256), 0, s, a, (const R*)F1, (const R*)F2, (const R*)F3, fe);
+    return launch_status("m3p_mhd_advance_ct");
+}
+
+template int mhd_advance<double>(MhdArgs, char*, hipStream_t);
+template int mhd_advance<float>(MhdArgs, char*, hipStream_t);
+
+} // namespace m3p

@@ -266,7 +266,7 @@ mhd_prim_kernel(MhdArgs a, R* P)
 }
 
 // dt from the kRateSlots partial maxima (one wave)
-__global__ void __launch_bounds__(64) mhd_dt_kernel(MhdArgs a)
+static __global__ void __launch_bounds__(64) mhd_dt_kernel(MhdArgs a)
 {
     double vmax = wave_max(__longlong_as_double((long long)a.vmax[threadIdx.x]));
     if (threadIdx.x == 0) *a.dt = a.cfl * a.dl / vmax;
@@ -660,7 +660,7 @@ mhd_face_b_kernel(MhdArgs a, FaceE<R> fe)
 }
 
 // max over cells of |div B| = |d1(B1) + d2(B2) + d3(B3)| (divergence_difference)
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 mhd_divb_kernel(const double* b1, const double* b2, const double* b3, int N, unsigned long long* slot)
 {
     __shared__ double part[4];

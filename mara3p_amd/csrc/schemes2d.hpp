@@ -315,9 +315,9 @@ struct MinidiskScheme {
 };
 
 // ---------------------------------------------------------------------------
-// locally_isothermal::updated_conserved with zero viscosity: the face sound
-// speeds squared come from arrays cs2x [(n+1)][n] and cs2y [n][(n+1)]
-// evaluated by the caller at the face coordinates (the reference evaluates a
+// locally_isothermal::updated_conserved: the face sound speeds squared come
+// from arrays cs2x [nblk][(n+1)][n] and cs2y [nblk][n][(n+1)] evaluated by the
+// caller at the face coordinates of each block (the reference evaluates a
 // host std::function per face, src/module_locally_isothermal.cpp:115-121).
 struct IsoTableParams {
     const double* cs2x;
@@ -350,9 +350,11 @@ struct IsoTableScheme {
 
     template<int AXIS, int H, int TJ>
     __device__ static __forceinline__ void face(const R* pl, const R* pr, const Params& prm,
-                                                const TileData<H, TJ>&, const TileCtx&, int fi, int fj, R* f)
+                                                const TileData<H, TJ>&, const TileCtx& tc, int fi, int fj, R* f)
     {
-        R cs2 = R(AXIS == 1 ? prm.cs2x[(size_t)fi * prm.n + fj] : prm.cs2y[(size_t)fi * (prm.n + 1) + fj]);
+        const size_t per_block = (size_t)(prm.n + 1) * prm.n;
+        R cs2 = R(AXIS == 1 ? prm.cs2x[tc.b * per_block + (size_t)fi * prm.n + fj]
+                            : prm.cs2y[tc.b * per_block + (size_t)fi * (prm.n + 1) + fj]);
         Iso2d<R>::template hlle<AXIS>(pl, pr, cs2, f);
     }
 

@@ -15,17 +15,29 @@
 // std::invalid_argument are thrown with the reference's message.
 //
 // Each per-block call stages its arrays to the device and back (the
-// compatibility path).  Device-resident meshes use the whole-mesh stage entry
-// points of the C ABI directly (m3p_euler2d_stage, m3p_minidisk_stage, ...).
+// compatibility path) on the stream given as its last, optional argument
+// (default: the null stream).  The per-block functions may be called from
+// any number of threads at once, as the reference's thread pool calls
+// updated_conserved (src/parallel_computable.cpp:248-296): the staging
+// buffers belong to the calling thread and the library reports each call's
+// kernel errors to that call.  The whole-mesh functions (updated_solution,
+// updated_conserved_tree) upload a tree once and run one fused stage launch
+// over all its blocks.  Device-resident meshes use the whole-mesh stage
+// entry points of the C ABI directly (m3p_euler2d_stage, m3p_minidisk_stage,
+// ...).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <limits>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 #include "mara3p_amd.h"
 
@@ -110,6 +122,18 @@ struct block_index_t
     std::array<unsigned long, Rank> coordinates = {};
 };
 
+// Ordering by level, then coordinates (keys of the block trees below).
+template<unsigned long R>
+bool operator<(const block_index_t<R>& a, const block_index_t<R>& b)
+{
+    return std::tie(a.level, a.coordinates) < std::tie(b.level, b.coordinates);
+}
+template<unsigned long R>
+bool operator==(const block_index_t<R>& a, const block_index_t<R>& b)
+{
+    return a.level == b.level && a.coordinates == b.coordinates;
+}
+
 template<unsigned long R>
 block_index_t<R> next_on(block_index_t<R> i, std::size_t a)
 {
@@ -130,6 +154,7 @@ struct geometry_t
     double domain_size = 1.0;
     int block_size = 64;
     double cell_spacing(block_index_t<1> b) const { return domain_size / double(block_size) / double(1 << b.level); }
+    std::size_t cells_per_block() const { return std::size_t(block_size); }
 };
 }
 
@@ -139,6 +164,7 @@ struct geometry_t
     double domain_size = 1.0;
     int block_size = 64;
     double cell_spacing(block_index_t<2> b) const { return domain_size / double(block_size) / double(1 << b.level); }
+    std::size_t cells_per_block() const { return std::size_t(block_size) * std::size_t(block_size); }
     // face_coordinates, src/mesh_geometry.cpp:125-146: (x, y) of the axis-oriented faces
     std::array<double, 2> face_coordinate(block_index_t<2> b, unsigned long axis, std::size_t i, std::size_t j) const
     {
@@ -158,38 +184,239 @@ struct geometry_t
 } // namespace mesh
 
 // ---------------------------------------------------------------------------
-// Device staging helpers
-struct device_buffer
+// Staging.  A call copies its arrays to the device and back through numbered
+// staging slots of the calling thread (device buffers per device, host
+// buffers), which grow as needed and are reused by the thread's later calls:
+// concurrent callers share nothing, and a warm caller allocates nothing.
+namespace detail {
+
+struct device_staging
 {
-    double* ptr = nullptr;
-    explicit device_buffer(std::size_t count)
+    std::vector<double*> buf;
+    std::vector<std::size_t> cap;
+    device_staging() = default;
+    device_staging(const device_staging&) = delete;
+    device_staging& operator=(const device_staging&) = delete;
+    ~device_staging() { for (double* p : buf) if (p) (void)hipFree(p); }
+    double* get(std::size_t slot, std::size_t count)
     {
-        hip_check(hipMalloc(&ptr, count * sizeof(double)), "hipMalloc");
+        if (slot >= buf.size()) { buf.resize(slot + 1, nullptr); cap.resize(slot + 1, 0); }
+        if (cap[slot] < count) {
+            if (buf[slot]) hip_check(hipFree(buf[slot]), "hipFree");
+            buf[slot] = nullptr;
+            cap[slot] = 0;
+            hip_check(hipMalloc(&buf[slot], count * sizeof(double)), "hipMalloc");
+            cap[slot] = count;
+        }
+        return buf[slot];
     }
-    ~device_buffer() { if (ptr) (void)hipFree(ptr); }
-    device_buffer(const device_buffer&) = delete;
-    device_buffer& operator=(const device_buffer&) = delete;
 };
 
-// Logical tuples [zones][NC] (host) -> SoA [NC][zones] (device), component by
-// component, never a memcpy of the tuple layout.
-template<std::size_t NC>
-inline void upload_soa(const std::array<double, NC>* src, std::size_t zones, double* dst)
+// Device buffer `slot` of the calling thread, at least `count` doubles.
+inline double* device_slot(std::size_t slot, std::size_t count)
 {
-    std::vector<double> tmp(NC * zones);
-    for (std::size_t z = 0; z < zones; ++z)
-        for (std::size_t q = 0; q < NC; ++q) tmp[q * zones + z] = src[z][q];
-    hip_check(hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    thread_local std::map<int, device_staging> per_device;
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    return per_device[dev].get(slot, std::max<std::size_t>(count, 1));
 }
 
-template<std::size_t NC>
-inline void download_soa(const double* src, std::size_t zones, std::array<double, NC>* dst)
+// Host buffer `slot` of the calling thread, at least `count` doubles.
+inline double* host_slot(std::size_t slot, std::size_t count)
 {
-    std::vector<double> tmp(NC * zones);
-    hip_check(hipMemcpy(tmp.data(), src, tmp.size() * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    for (std::size_t z = 0; z < zones; ++z)
-        for (std::size_t q = 0; q < NC; ++q) dst[z][q] = tmp[q * zones + z];
+    thread_local std::vector<std::vector<double>> bufs;
+    if (slot >= bufs.size()) bufs.resize(slot + 1);
+    if (bufs[slot].size() < count) bufs[slot].resize(count);
+    return bufs[slot].data();
 }
+
+} // namespace detail
+
+// Logical tuples [zones][NC] (host) -> SoA [NC][zones] in device slot `slot`,
+// component by component (never a memcpy of the tuple layout).
+template<std::size_t NC>
+inline double* upload_soa(const std::array<double, NC>* src, std::size_t zones, std::size_t slot, hipStream_t stream)
+{
+    double* h = detail::host_slot(slot, NC * zones);
+    for (std::size_t z = 0; z < zones; ++z)
+        for (std::size_t q = 0; q < NC; ++q) h[q * zones + z] = src[z][q];
+    double* d = detail::device_slot(slot, NC * zones);
+    hip_check(hipMemcpyAsync(d, h, NC * zones * sizeof(double), hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+    return d;
+}
+
+inline double* upload(const double* src, std::size_t count, std::size_t slot, hipStream_t stream)
+{
+    double* h = detail::host_slot(slot, count);
+    std::copy(src, src + count, h);
+    double* d = detail::device_slot(slot, count);
+    hip_check(hipMemcpyAsync(d, h, count * sizeof(double), hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+    return d;
+}
+
+// SoA [NC][zones] (device) -> logical tuples; waits for the stream.
+template<std::size_t NC>
+inline void download_soa(const double* src, std::size_t zones, std::array<double, NC>* dst, std::size_t slot,
+                         hipStream_t stream)
+{
+    double* h = detail::host_slot(slot, NC * zones);
+    hip_check(hipMemcpyAsync(h, src, NC * zones * sizeof(double), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    for (std::size_t z = 0; z < zones; ++z)
+        for (std::size_t q = 0; q < NC; ++q) dst[z][q] = h[q * zones + z];
+}
+
+inline void download(const double* src, std::size_t count, double* dst, std::size_t slot, hipStream_t stream)
+{
+    double* h = detail::host_slot(slot, count);
+    hip_check(hipMemcpyAsync(h, src, count * sizeof(double), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    std::copy(h, h + count, dst);
+}
+
+// ---------------------------------------------------------------------------
+// Block trees: the leaves of a bsp::shared_tree as an ordered map from block
+// index to block array (src/core_bqo_tree.hpp; the reference stores lazy
+// computables there, this API the evaluated arrays), and the topology (the
+// leaf indexes).
+namespace bsp {
+template<typename T, unsigned long Rank>
+using tree_t = std::map<mesh::block_index_t<Rank>, T>;
+
+template<typename T, unsigned long Rank>
+std::vector<mesh::block_index_t<Rank>> indexes(const tree_t<T, Rank>& tree)
+{
+    std::vector<mesh::block_index_t<Rank>> out;
+    for (const auto& kv : tree) out.push_back(kv.first);
+    return out;
+}
+} // namespace bsp
+
+namespace detail {
+
+// Leaf b's neighbour table row: next_on / prev_on of each axis (periodic on
+// its level), as indexes into `leaves`; all leaves must share one level.
+template<unsigned long R>
+std::vector<int> uniform_neighbours(const std::vector<mesh::block_index_t<R>>& leaves)
+{
+    std::map<mesh::block_index_t<R>, int> at;
+    for (std::size_t k = 0; k < leaves.size(); ++k) at[leaves[k]] = int(k);
+    std::vector<int> nbr(2 * R * leaves.size());
+    for (std::size_t k = 0; k < leaves.size(); ++k) {
+        if (leaves[k].level != leaves[0].level)
+            throw std::invalid_argument("mara3p: this update needs a tree whose leaves share one level");
+        for (unsigned long a = 0; a < R; ++a) {
+            auto lo = at.find(mesh::prev_on(leaves[k], a)), hi = at.find(mesh::next_on(leaves[k], a));
+            if (lo == at.end() || hi == at.end())
+                throw std::invalid_argument("mara3p: the tree does not cover its level (a neighbour block is missing)");
+            nbr[2 * R * k + 2 * a] = lo->second;
+            nbr[2 * R * k + 2 * a + 1] = hi->second;
+        }
+    }
+    return nbr;
+}
+
+// Guard-fill recipes of a 2:1-balanced quad-tree for m3p_amr_euler2d_stage:
+// per leaf and direction (-x, +x, -y, +y), the next_on / prev_on neighbour
+// as amr::get_or_create_block (src/mesh_amr.hpp:300-328) finds it -- a leaf
+// of the same level (kind 0), the leaf containing it one level up (kind 1,
+// with the orthant), or four leaves one level down (kind 2, children in the
+// order (0,0), (1,0), (0,1), (1,1)).
+inline std::vector<int> quadtree_recipes(const std::vector<mesh::block_index_t<2>>& leaves)
+{
+    using idx = mesh::block_index_t<2>;
+    std::map<idx, int> at;
+    for (std::size_t k = 0; k < leaves.size(); ++k) at[leaves[k]] = int(k);
+    std::vector<int> rec(leaves.size() * 4 * 8, -1);
+    for (std::size_t k = 0; k < leaves.size(); ++k)
+        for (int d = 0; d < 4; ++d) {
+            const unsigned long axis = d / 2;
+            const idx nb = d % 2 ? mesh::next_on(leaves[k], axis) : mesh::prev_on(leaves[k], axis);
+            int* r = &rec[(k * 4 + d) * 8];
+            auto same = at.find(nb);
+            if (same != at.end()) { r[0] = 0; r[1] = same->second; continue; }
+            if (nb.level > 0) {
+                idx parent{ nb.level - 1, { nb.coordinates[0] >> 1, nb.coordinates[1] >> 1 } };
+                auto up = at.find(parent);
+                if (up != at.end()) {
+                    r[0] = 1; r[1] = up->second;
+                    r[6] = int(nb.coordinates[0] & 1); r[7] = int(nb.coordinates[1] & 1);
+                    continue;
+                }
+            }
+            r[0] = 2;
+            for (int c = 0; c < 4; ++c) {
+                idx child{ nb.level + 1, { 2 * nb.coordinates[0] + (c & 1), 2 * nb.coordinates[1] + (c >> 1) } };
+                auto down = at.find(child);
+                if (down == at.end()) throw std::invalid_argument("mara3p: the quad-tree is not 2:1 balanced");
+                r[2 + c] = down->second;
+            }
+        }
+    return rec;
+}
+
+// Device copy of an int table (caller's slot, reinterpreted storage).
+inline const int* upload_ints(const std::vector<int>& v, std::size_t slot, hipStream_t stream)
+{
+    const std::size_t words = (v.size() * sizeof(int) + sizeof(double) - 1) / sizeof(double);
+    double* d = device_slot(slot, words);
+    double* h = host_slot(slot, words);
+    std::memcpy(h, v.data(), v.size() * sizeof(int));
+    hip_check(hipMemcpyAsync(d, h, v.size() * sizeof(int), hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+    return reinterpret_cast<const int*>(d);
+}
+
+// Uploads the blocks of a tree as one [nblk][NC][zones] SoA array.
+template<std::size_t NC, unsigned long R, typename A>
+double* upload_tree(const bsp::tree_t<A, R>& tree, std::size_t zones, std::size_t slot, hipStream_t stream)
+{
+    const std::size_t nblk = tree.size();
+    double* h = host_slot(slot, nblk * NC * zones);
+    std::size_t b = 0;
+    for (const auto& kv : tree) {
+        if (kv.second.size() != zones) throw std::invalid_argument("mara3p: tree blocks must all have block_size zones per axis");
+        const std::array<double, NC>* src = kv.second.data();
+        for (std::size_t z = 0; z < zones; ++z)
+            for (std::size_t q = 0; q < NC; ++q) h[(b * NC + q) * zones + z] = src[z][q];
+        ++b;
+    }
+    double* d = device_slot(slot, nblk * NC * zones);
+    hip_check(hipMemcpyAsync(d, h, nblk * NC * zones * sizeof(double), hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+    return d;
+}
+
+// The inverse: a tree with the keys of `like` from a device SoA array.
+template<std::size_t NC, unsigned long R, typename A>
+bsp::tree_t<A, R> download_tree(const double* d, const bsp::tree_t<A, R>& like, std::size_t zones, std::size_t slot,
+                                hipStream_t stream)
+{
+    const std::size_t nblk = like.size();
+    double* h = host_slot(slot, nblk * NC * zones);
+    hip_check(hipMemcpyAsync(h, d, nblk * NC * zones * sizeof(double), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    bsp::tree_t<A, R> out;
+    std::size_t b = 0;
+    for (const auto& kv : like) {
+        A a(kv.second.shape());
+        std::array<double, NC>* dst = a.data();
+        for (std::size_t z = 0; z < zones; ++z)
+            for (std::size_t q = 0; q < NC; ++q) dst[z][q] = h[(b * NC + q) * zones + z];
+        out.emplace(kv.first, std::move(a));
+        ++b;
+    }
+    return out;
+}
+
+// Waits for the stream and reports the kernel errors latched on it.
+inline void check_stream(hipStream_t stream)
+{
+    int code = 0, block = 0, zone = 0;
+    check(m3p_check_errors(stream, &code, &block, &zone));
+    if (code == M3P_ERR_NEGATIVE_DENSITY) throw std::invalid_argument("euler::recover_primitive (negative density)");
+    if (code) throw std::runtime_error("mara3p: kernel error " + std::to_string(code));
+}
+
+} // namespace detail
 
 // ---------------------------------------------------------------------------
 namespace euler {
@@ -209,63 +436,132 @@ namespace euler2d {
 using conserved_array_t = nd::shared_array<euler::conserved_density_t, 2>;
 using primitive_array_t = nd::shared_array<euler::primitive_t, 2>;
 using mesh_geometry_t   = mesh::cartesian_2d::geometry_t;
+using conserved_tree_t  = bsp::tree_t<conserved_array_t, 2>;
+using mesh_topology_t   = std::vector<mesh::block_index_t<2>>;
+
+// src/module_euler.hpp:181-186
+struct solution_t
+{
+    double iteration = 0.0;
+    double time = 0.0;
+    conserved_tree_t conserved;
+};
 
 // src/module_euler.cpp:240-245
-inline primitive_array_t recover_primitive_array(conserved_array_t uc, double gamma_law_index)
+inline primitive_array_t recover_primitive_array(conserved_array_t uc, double gamma_law_index, hipStream_t stream = nullptr)
 {
     const std::size_t nz = uc.size();
-    device_buffer du(5 * nz), dp(5 * nz);
-    upload_soa(uc.data(), nz, du.ptr);
-    check(m3p_euler_recover_primitive_array(du.ptr, dp.ptr, long(nz), gamma_law_index, nullptr));
+    double* du = upload_soa(uc.data(), nz, 0, stream);
+    double* dp = detail::device_slot(1, 5 * nz);
+    check(m3p_euler_recover_primitive_array(du, dp, long(nz), gamma_law_index, stream));
     primitive_array_t p(uc.shape());
-    download_soa(dp.ptr, nz, p.data());
+    download_soa(dp, nz, p.data(), 1, stream);
     return p;
 }
 
 // src/module_euler.cpp:251-257
-inline primitive_array_t estimate_gradient(primitive_array_t pc, unsigned long axis, double plm_theta)
+inline primitive_array_t estimate_gradient(primitive_array_t pc, unsigned long axis, double plm_theta,
+                                           hipStream_t stream = nullptr)
 {
     const std::size_t ni = pc.shape(0), nj = pc.shape(1), nz = pc.size();
     if (axis > 1 || (axis == 0 ? ni : nj) < 3) throw std::invalid_argument("nd::select (selection out of range)");
     auto oshape = nd::uivec(axis == 0 ? ni - 2 : ni, axis == 1 ? nj - 2 : nj);
     const std::size_t no = oshape[0] * oshape[1];
-    device_buffer dp(5 * nz), dg(5 * no);
-    upload_soa(pc.data(), nz, dp.ptr);
-    check(m3p_plm_gradient_2d(dp.ptr, dg.ptr, 5, int(ni), int(nj), int(axis), plm_theta, nullptr));
+    double* dp = upload_soa(pc.data(), nz, 0, stream);
+    double* dg = detail::device_slot(1, 5 * no);
+    check(m3p_plm_gradient_2d(dp, dg, 5, int(ni), int(nj), int(axis), plm_theta, stream));
     primitive_array_t g(oshape);
-    download_soa(dg.ptr, no, g.data());
+    download_soa(dg, no, g.data(), 1, stream);
     return g;
 }
 
 // src/module_euler.cpp:263-298 (declared src/module_euler.hpp:182-190)
 inline conserved_array_t updated_conserved(conserved_array_t uc, primitive_array_t pe, double /*time*/, double dt,
                                            mesh_geometry_t mesh_geometry, mesh::block_index_t<2> block,
-                                           double plm_theta, double gamma_law_index)
+                                           double plm_theta, double gamma_law_index, hipStream_t stream = nullptr)
 {
     const std::size_t n = uc.shape(0);
     if (uc.shape(1) != n || pe.shape(0) != n + 4 || pe.shape(1) != n + 4)
         throw std::invalid_argument("euler2d::updated_conserved (pe must be uc extended by 2 guard zones)");
-    device_buffer du(5 * n * n), dp(5 * (n + 4) * (n + 4)), dout(5 * n * n);
-    upload_soa(uc.data(), n * n, du.ptr);
-    upload_soa(pe.data(), (n + 4) * (n + 4), dp.ptr);
-    check(m3p_euler2d_updated_conserved(du.ptr, dp.ptr, dout.ptr, int(n), dt, mesh_geometry.cell_spacing(block),
-                                        plm_theta, gamma_law_index, 0, M3P_FP64, nullptr));
+    double* du = upload_soa(uc.data(), n * n, 0, stream);
+    double* dp = upload_soa(pe.data(), (n + 4) * (n + 4), 1, stream);
+    double* dout = detail::device_slot(2, 5 * n * n);
+    check(m3p_euler2d_updated_conserved(du, dp, dout, int(n), dt, mesh_geometry.cell_spacing(block),
+                                        plm_theta, gamma_law_index, 0, M3P_FP64, stream));
     conserved_array_t out(uc.shape());
-    download_soa(dout.ptr, n * n, out.data());
+    download_soa(dout, n * n, out.data(), 2, stream);
     return out;
 }
 
+// updated_solution, src/module_euler.cpp:304-323: one forward-Euler update of
+// every block (guard zones from the neighbours, amr::extend_block over
+// amr::get_or_create_block), iteration + 1, time + dt.  A tree whose leaves
+// share one level runs the uniform-mesh stage; a 2:1-balanced tree with
+// level jumps runs the quad-tree stage (injection from coarser and 2 x 2
+// means of finer neighbours, as the reference).
+inline solution_t updated_solution(solution_t solution, double dt, mesh_geometry_t mesh_geometry,
+                                   double plm_theta, double gamma_law_index, hipStream_t stream = nullptr)
+{
+    const auto leaves = bsp::indexes(solution.conserved);
+    if (leaves.empty()) throw std::invalid_argument("euler2d::updated_solution (empty tree)");
+    const std::size_t n = std::size_t(mesh_geometry.block_size), zones = n * n;
+    bool uniform = true;
+    for (const auto& b : leaves) uniform &= b.level == leaves[0].level;
+    double* du = detail::upload_tree<5>(solution.conserved, zones, 0, stream);
+    double* dout = detail::device_slot(1, leaves.size() * 5 * zones);
+    if (uniform) {
+        const int* nbr = detail::upload_ints(detail::uniform_neighbours(leaves), 2, stream);
+        check(m3p_euler2d_stage(du, nullptr, dout, nbr, int(leaves.size()), int(n), dt, nullptr,
+                                mesh_geometry.cell_spacing(leaves[0]), 0.5, plm_theta, gamma_law_index, 0, M3P_FP64, stream));
+    } else {
+        const int* rec = detail::upload_ints(detail::quadtree_recipes(leaves), 2, stream);
+        std::vector<double> dl(leaves.size());
+        for (std::size_t k = 0; k < leaves.size(); ++k) dl[k] = mesh_geometry.cell_spacing(leaves[k]);
+        double* ddl = upload(dl.data(), dl.size(), 3, stream);
+        check(m3p_amr_euler2d_stage(du, nullptr, dout, nullptr, rec, ddl, int(leaves.size()), int(n), dt, nullptr, 0.5,
+                                    plm_theta, gamma_law_index, 0, M3P_FP64, stream));
+    }
+    detail::check_stream(stream);
+    return { solution.iteration + 1, solution.time + dt,
+             detail::download_tree<5>(dout, solution.conserved, zones, 4, stream) };
+}
+
+// src/module_euler.cpp:329-337
+inline double smallest_cell_size(mesh_topology_t mesh_topology, mesh_geometry_t mesh_geometry)
+{
+    double dl = std::numeric_limits<double>::max();
+    for (const auto& b : mesh_topology) dl = std::min(dl, mesh_geometry.cell_spacing(b));
+    return dl;
+}
+
+// src/module_euler.cpp:343-348
+inline std::size_t total_zones(mesh_topology_t mesh_topology, mesh_geometry_t mesh_geometry)
+{
+    return mesh_topology.size() * mesh_geometry.cells_per_block();
+}
+
 // weighted_sum over blocks (src/module_euler.cpp:353-360 -> src/mesh_amr.hpp:455-467)
-inline conserved_array_t weighted_sum(conserved_array_t s, conserved_array_t t, double b)
+inline conserved_array_t weighted_sum(conserved_array_t s, conserved_array_t t, double b, hipStream_t stream = nullptr)
 {
     const std::size_t nz = s.size();
     if (t.size() != nz) throw std::invalid_argument("euler2d::weighted_sum (shape mismatch)");
-    device_buffer ds(5 * nz), dt_(5 * nz), dout(5 * nz);
-    upload_soa(s.data(), nz, ds.ptr);
-    upload_soa(t.data(), nz, dt_.ptr);
-    check(m3p_weighted_sum(ds.ptr, dt_.ptr, dout.ptr, long(5 * nz), b, nullptr));
+    double* ds = upload_soa(s.data(), nz, 0, stream);
+    double* dt_ = upload_soa(t.data(), nz, 1, stream);
+    double* dout = detail::device_slot(2, 5 * nz);
+    check(m3p_weighted_sum(ds, dt_, dout, long(5 * nz), b, stream));
     conserved_array_t out(s.shape());
-    download_soa(dout.ptr, nz, out.data());
+    download_soa(dout, nz, out.data(), 2, stream);
+    return out;
+}
+
+// weighted_sum of solutions, src/module_euler.cpp:353-360: iteration, time
+// and every block as s * b + t * (1 - b).
+inline solution_t weighted_sum(solution_t s, solution_t t, double b, hipStream_t stream = nullptr)
+{
+    if (bsp::indexes(s.conserved) != bsp::indexes(t.conserved))
+        throw std::invalid_argument("euler2d::weighted_sum (the trees differ)");
+    solution_t out{ s.iteration * b + t.iteration * (1 - b), s.time * b + t.time * (1 - b), {} };
+    for (const auto& kv : s.conserved) out.conserved.emplace(kv.first, weighted_sum(kv.second, t.conserved.at(kv.first), b, stream));
     return out;
 }
 } // namespace euler2d
@@ -275,39 +571,112 @@ namespace euler1d {
 using conserved_array_t = nd::shared_array<euler::conserved_density_t, 1>;
 using primitive_array_t = nd::shared_array<euler::primitive_t, 1>;
 using mesh_geometry_t   = mesh::cartesian_1d::geometry_t;
+using conserved_tree_t  = bsp::tree_t<conserved_array_t, 1>;
+using mesh_topology_t   = std::vector<mesh::block_index_t<1>>;
 // The reference's boundary_condition_t is a std::function applied to pe; the
 // outflow condition (src/mara.cpp:123-143) is the one the module supports
 // natively here: it is selected by block position, as in the reference.
 enum class boundary_condition_t { periodic, outflow };
 
-inline primitive_array_t recover_primitive_array(conserved_array_t uc, double gamma_law_index)
+// src/module_euler.hpp:66-71
+struct solution_t
+{
+    double iteration = 0.0;
+    double time = 0.0;
+    conserved_tree_t conserved;
+};
+
+inline primitive_array_t recover_primitive_array(conserved_array_t uc, double gamma_law_index, hipStream_t stream = nullptr)
 {
     const std::size_t nz = uc.size();
-    device_buffer du(5 * nz), dp(5 * nz);
-    upload_soa(uc.data(), nz, du.ptr);
-    check(m3p_euler_recover_primitive_array(du.ptr, dp.ptr, long(nz), gamma_law_index, nullptr));
+    double* du = upload_soa(uc.data(), nz, 0, stream);
+    double* dp = detail::device_slot(1, 5 * nz);
+    check(m3p_euler_recover_primitive_array(du, dp, long(nz), gamma_law_index, stream));
     primitive_array_t p(uc.shape());
-    download_soa(dp.ptr, nz, p.data());
+    download_soa(dp, nz, p.data(), 1, stream);
     return p;
 }
 
 // src/module_euler.cpp:107-135 (declared src/module_euler.hpp:98-107)
 inline conserved_array_t updated_conserved(conserved_array_t uc, primitive_array_t pe, double /*time*/, double dt,
                                            mesh_geometry_t mesh_geometry, boundary_condition_t bc,
-                                           mesh::block_index_t<1> block, double plm_theta, double gamma_law_index)
+                                           mesh::block_index_t<1> block, double plm_theta, double gamma_law_index,
+                                           hipStream_t stream = nullptr)
 {
     const std::size_t n = uc.shape(0);
     if (pe.shape(0) != n + 4) throw std::invalid_argument("euler1d::updated_conserved (pe must be uc extended by 2 guard zones)");
     const bool outflow = bc == boundary_condition_t::outflow;
     const int lower = outflow && block.coordinates[0] == 0;
     const int upper = outflow && !lower && block.coordinates[0] == (1ul << block.level) - 1;
-    device_buffer du(5 * n), dp(5 * (n + 4)), dout(5 * n);
-    upload_soa(uc.data(), n, du.ptr);
-    upload_soa(pe.data(), n + 4, dp.ptr);
-    check(m3p_euler1d_updated_conserved(du.ptr, dp.ptr, dout.ptr, int(n), dt, mesh_geometry.cell_spacing(block),
-                                        plm_theta, gamma_law_index, lower, upper, M3P_FP64, nullptr));
+    double* du = upload_soa(uc.data(), n, 0, stream);
+    double* dp = upload_soa(pe.data(), n + 4, 1, stream);
+    double* dout = detail::device_slot(2, 5 * n);
+    check(m3p_euler1d_updated_conserved(du, dp, dout, int(n), dt, mesh_geometry.cell_spacing(block),
+                                        plm_theta, gamma_law_index, lower, upper, M3P_FP64, stream));
     conserved_array_t out(uc.shape());
-    download_soa(dout.ptr, n, out.data());
+    download_soa(dout, n, out.data(), 2, stream);
+    return out;
+}
+
+// updated_solution, src/module_euler.cpp:141-161: one forward-Euler update of
+// every block of a tree whose leaves share one level, the chain's ends
+// closed by the boundary condition (outflow: the first and last block copy
+// their edge zone into the guard zones, src/mara.cpp:123-143; periodic: the
+// ends wrap), iteration + 1, time + dt.
+inline solution_t updated_solution(solution_t solution, double dt, mesh_geometry_t mesh_geometry,
+                                   boundary_condition_t bc, double plm_theta, double gamma_law_index,
+                                   hipStream_t stream = nullptr)
+{
+    const auto leaves = bsp::indexes(solution.conserved);
+    if (leaves.empty()) throw std::invalid_argument("euler1d::updated_solution (empty tree)");
+    const std::size_t n = std::size_t(mesh_geometry.block_size);
+    std::vector<int> nbr = detail::uniform_neighbours(leaves);
+    if (bc == boundary_condition_t::outflow) {
+        nbr.front() = -1;    // prev of the first block
+        nbr.back() = -1;     // next of the last block
+    }
+    double* du = detail::upload_tree<5>(solution.conserved, n, 0, stream);
+    double* dout = detail::device_slot(1, leaves.size() * 5 * n);
+    const int* dn = detail::upload_ints(nbr, 2, stream);
+    check(m3p_euler1d_stage(du, nullptr, dout, dn, int(leaves.size()), int(n), dt, mesh_geometry.cell_spacing(leaves[0]),
+                            0.5, plm_theta, gamma_law_index, M3P_FP64, stream));
+    detail::check_stream(stream);
+    return { solution.iteration + 1, solution.time + dt,
+             detail::download_tree<5>(dout, solution.conserved, n, 3, stream) };
+}
+
+// src/module_euler.cpp:166-175
+inline double smallest_cell_size(mesh_topology_t mesh_topology, mesh_geometry_t mesh_geometry)
+{
+    double dl = std::numeric_limits<double>::max();
+    for (const auto& b : mesh_topology) dl = std::min(dl, mesh_geometry.cell_spacing(b));
+    return dl;
+}
+
+// src/module_euler.cpp:181-186
+inline std::size_t total_zones(mesh_topology_t mesh_topology, mesh_geometry_t mesh_geometry)
+{
+    return mesh_topology.size() * mesh_geometry.cells_per_block();
+}
+
+// weighted_sum of solutions, src/module_euler.cpp:191-198
+inline solution_t weighted_sum(solution_t s, solution_t t, double b, hipStream_t stream = nullptr)
+{
+    if (bsp::indexes(s.conserved) != bsp::indexes(t.conserved))
+        throw std::invalid_argument("euler1d::weighted_sum (the trees differ)");
+    solution_t out{ s.iteration * b + t.iteration * (1 - b), s.time * b + t.time * (1 - b), {} };
+    for (const auto& kv : s.conserved) {
+        const conserved_array_t& x = kv.second;
+        const conserved_array_t& y = t.conserved.at(kv.first);
+        const std::size_t nz = x.size();
+        double* ds = upload_soa(x.data(), nz, 0, stream);
+        double* dt_ = upload_soa(y.data(), nz, 1, stream);
+        double* dout = detail::device_slot(2, 5 * nz);
+        check(m3p_weighted_sum(ds, dt_, dout, long(5 * nz), b, stream));
+        conserved_array_t w(x.shape());
+        download_soa(dout, nz, w.data(), 2, stream);
+        out.conserved.emplace(kv.first, std::move(w));
+    }
     return out;
 }
 } // namespace euler1d
@@ -317,19 +686,36 @@ namespace locally_isothermal {
 using conserved_array_t     = nd::shared_array<iso2d::conserved_density_t, 2>;
 using primitive_array_t     = nd::shared_array<iso2d::primitive_t, 2>;
 using mesh_geometry_t       = mesh::cartesian_2d::geometry_t;
+using conserved_tree_t      = bsp::tree_t<conserved_array_t, 2>;
+using mesh_topology_t       = std::vector<mesh::block_index_t<2>>;
 using coords_t              = std::array<double, 2>;
 using temperature_mapping_t = std::function<double(coords_t)>;
 
-inline primitive_array_t recover_primitive_array(conserved_array_t uc)
+inline primitive_array_t recover_primitive_array(conserved_array_t uc, hipStream_t stream = nullptr)
 {
     const std::size_t nz = uc.size();
-    device_buffer du(3 * nz), dp(3 * nz);
-    upload_soa(uc.data(), nz, du.ptr);
-    check(m3p_iso2d_recover_primitive_array(du.ptr, dp.ptr, long(nz), nullptr));
+    double* du = upload_soa(uc.data(), nz, 0, stream);
+    double* dp = detail::device_slot(1, 3 * nz);
+    check(m3p_iso2d_recover_primitive_array(du, dp, long(nz), stream));
     primitive_array_t p(uc.shape());
-    download_soa(dp.ptr, nz, p.data());
+    download_soa(dp, nz, p.data(), 1, stream);
     return p;
 }
+
+namespace detail_ {
+// The temperature mapping at the faces of block b (the reference evaluates it
+// per face, src/module_locally_isothermal.cpp:115-121): cx [(n+1)][n] on the
+// x faces, cy [n][(n+1)] on the y faces.
+inline void face_temperatures(const mesh_geometry_t& g, mesh::block_index_t<2> b, const temperature_mapping_t& T,
+                              double* cx, double* cy)
+{
+    const std::size_t n = std::size_t(g.block_size);
+    for (std::size_t i = 0; i <= n; ++i)
+        for (std::size_t j = 0; j < n; ++j) cx[i * n + j] = T(g.face_coordinate(b, 0, i, j));
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j = 0; j <= n; ++j) cy[i * (n + 1) + j] = T(g.face_coordinate(b, 1, i, j));
+}
+} // namespace detail_
 
 // src/module_locally_isothermal.cpp:182-212.  The temperature mapping is a
 // host callback in the reference too; it is evaluated here at the face
@@ -338,28 +724,70 @@ inline primitive_array_t recover_primitive_array(conserved_array_t uc)
 inline conserved_array_t updated_conserved(conserved_array_t uc, primitive_array_t pe, double /*time*/, double dt,
                                            mesh_geometry_t mesh_geometry, mesh::block_index_t<2> block,
                                            double kinematic_viscosity, temperature_mapping_t temperature,
-                                           double /*plm_theta*/)
+                                           double /*plm_theta*/, hipStream_t stream = nullptr)
 {
     const std::size_t n = uc.shape(0);
     if (pe.shape(0) != n + 4 || pe.shape(1) != n + 4)
         throw std::invalid_argument("locally_isothermal::updated_conserved (pe must be uc extended by 2 guard zones)");
     if (kinematic_viscosity < 0.0)
         throw std::invalid_argument("locally_isothermal::updated_conserved (negative kinematic viscosity)");
+    mesh_geometry_t g = mesh_geometry;
+    g.block_size = int(n);
     std::vector<double> cx((n + 1) * n), cy(n * (n + 1));
-    for (std::size_t i = 0; i <= n; ++i)
-        for (std::size_t j = 0; j < n; ++j) cx[i * n + j] = temperature(mesh_geometry.face_coordinate(block, 0, i, j));
-    for (std::size_t i = 0; i < n; ++i)
-        for (std::size_t j = 0; j <= n; ++j) cy[i * (n + 1) + j] = temperature(mesh_geometry.face_coordinate(block, 1, i, j));
-    device_buffer du(3 * n * n), dp(3 * (n + 4) * (n + 4)), dout(3 * n * n), dcx(cx.size()), dcy(cy.size());
-    upload_soa(uc.data(), n * n, du.ptr);
-    upload_soa(pe.data(), (n + 4) * (n + 4), dp.ptr);
-    hip_check(hipMemcpy(dcx.ptr, cx.data(), cx.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
-    hip_check(hipMemcpy(dcy.ptr, cy.data(), cy.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
-    check(m3p_locally_isothermal_updated_conserved(du.ptr, dp.ptr, dout.ptr, int(n), dt, mesh_geometry.cell_spacing(block),
-                                                   dcx.ptr, dcy.ptr, kinematic_viscosity, M3P_FP64, nullptr));
+    detail_::face_temperatures(g, block, temperature, cx.data(), cy.data());
+    double* du = upload_soa(uc.data(), n * n, 0, stream);
+    double* dp = upload_soa(pe.data(), (n + 4) * (n + 4), 1, stream);
+    double* dcx = upload(cx.data(), cx.size(), 3, stream);
+    double* dcy = upload(cy.data(), cy.size(), 4, stream);
+    double* dout = detail::device_slot(2, 3 * n * n);
+    check(m3p_locally_isothermal_updated_conserved(du, dp, dout, int(n), dt, mesh_geometry.cell_spacing(block),
+                                                   dcx, dcy, kinematic_viscosity, M3P_FP64, stream));
     conserved_array_t out(uc.shape());
-    download_soa(dout.ptr, n * n, out.data());
+    download_soa(dout, n * n, out.data(), 2, stream);
     return out;
+}
+
+// updated_conserved_tree, src/module_locally_isothermal.cpp:218-238: every
+// block of a tree whose leaves share one level, updated in one stage launch
+// (guard zones from the neighbours, periodic on the level).
+inline conserved_tree_t updated_conserved_tree(conserved_tree_t conserved, double /*time*/, double dt,
+                                               mesh_geometry_t mesh_geometry, temperature_mapping_t temperature,
+                                               double kinematic_viscosity, double /*plm_theta*/,
+                                               hipStream_t stream = nullptr)
+{
+    const auto leaves = bsp::indexes(conserved);
+    if (leaves.empty()) throw std::invalid_argument("locally_isothermal::updated_conserved_tree (empty tree)");
+    if (kinematic_viscosity < 0.0)
+        throw std::invalid_argument("locally_isothermal::updated_conserved_tree (negative kinematic viscosity)");
+    const std::size_t n = std::size_t(mesh_geometry.block_size), zones = n * n, faces = (n + 1) * n;
+    const std::vector<int> nbr = detail::uniform_neighbours(leaves);
+    std::vector<double> cx(leaves.size() * faces), cy(leaves.size() * faces);
+    for (std::size_t k = 0; k < leaves.size(); ++k)
+        detail_::face_temperatures(mesh_geometry, leaves[k], temperature, &cx[k * faces], &cy[k * faces]);
+    double* du = detail::upload_tree<3>(conserved, zones, 0, stream);
+    double* dout = detail::device_slot(1, leaves.size() * 3 * zones);
+    const int* dn = detail::upload_ints(nbr, 2, stream);
+    double* dcx = upload(cx.data(), cx.size(), 3, stream);
+    double* dcy = upload(cy.data(), cy.size(), 4, stream);
+    check(m3p_locally_isothermal_stage(du, nullptr, dout, dn, int(leaves.size()), int(n), dt, nullptr,
+                                       mesh_geometry.cell_spacing(leaves[0]), 0.5, dcx, dcy, kinematic_viscosity,
+                                       M3P_FP64, stream));
+    detail::check_stream(stream);
+    return detail::download_tree<3>(dout, conserved, zones, 5, stream);
+}
+
+// src/module_locally_isothermal.cpp:244-252
+inline double smallest_cell_size(mesh_topology_t mesh_topology, mesh_geometry_t mesh_geometry)
+{
+    double dl = std::numeric_limits<double>::max();
+    for (const auto& b : mesh_topology) dl = std::min(dl, mesh_geometry.cell_spacing(b));
+    return dl;
+}
+
+// src/module_locally_isothermal.cpp:258-263
+inline std::size_t total_zones(mesh_topology_t mesh_topology, mesh_geometry_t mesh_geometry)
+{
+    return mesh_topology.size() * mesh_geometry.cells_per_block();
 }
 } // namespace locally_isothermal
 
@@ -373,18 +801,19 @@ using solver_data_t = m3p_minidisk_data;
 
 // problems/minidisk_scheme.cpp:294-355 (declared problems/minidisk.hpp:164-170)
 inline conserved_array_t updated_conserved(conserved_array_t uc, primitive_array_t pe, double time, double dt,
-                                           mesh::block_index_t<2> block, solver_data_t solver_data)
+                                           mesh::block_index_t<2> block, solver_data_t solver_data,
+                                           hipStream_t stream = nullptr)
 {
     const std::size_t n = uc.shape(0);
     if (n != std::size_t(solver_data.block_size) || pe.shape(0) != n + 4 || pe.shape(1) != n + 4)
         throw std::invalid_argument("minidisk::updated_conserved (array shapes do not match block_size)");
-    device_buffer du(3 * n * n), dp(3 * (n + 4) * (n + 4)), dout(3 * n * n);
-    upload_soa(uc.data(), n * n, du.ptr);
-    upload_soa(pe.data(), (n + 4) * (n + 4), dp.ptr);
-    check(m3p_minidisk_updated_conserved(du.ptr, dp.ptr, dout.ptr, int(block.level), int(block.coordinates[0]),
-                                         int(block.coordinates[1]), time, dt, &solver_data, M3P_FP64, nullptr));
+    double* du = upload_soa(uc.data(), n * n, 0, stream);
+    double* dp = upload_soa(pe.data(), (n + 4) * (n + 4), 1, stream);
+    double* dout = detail::device_slot(2, 3 * n * n);
+    check(m3p_minidisk_updated_conserved(du, dp, dout, int(block.level), int(block.coordinates[0]),
+                                         int(block.coordinates[1]), time, dt, &solver_data, M3P_FP64, stream));
     conserved_array_t out(uc.shape());
-    download_soa(dout.ptr, n * n, out.data());
+    download_soa(dout, n * n, out.data(), 2, stream);
     return out;
 }
 } // namespace minidisk
@@ -409,33 +838,33 @@ namespace detail {
 using shape3 = nd::uivec_t<3>;
 inline std::size_t count(shape3 s) { return s[0] * s[1] * s[2]; }
 
-inline std::unique_ptr<device_buffer> up(const nd::shared_array<double, 3>& a)
+// Staging through the calling thread's slots (mara3p::detail), one slot per
+// array of a call.
+inline double* up(const nd::shared_array<double, 3>& a, std::size_t slot, hipStream_t stream)
 {
-    auto d = std::make_unique<device_buffer>(a.size());
-    hip_check(hipMemcpy(d->ptr, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
-    return d;
+    return upload(a.data(), a.size(), slot, stream);
 }
 
 template<std::size_t NC>
-inline std::unique_ptr<device_buffer> up(const nd::shared_array<std::array<double, NC>, 3>& a)
+inline double* up(const nd::shared_array<std::array<double, NC>, 3>& a, std::size_t slot, hipStream_t stream)
 {
-    auto d = std::make_unique<device_buffer>(a.size() * NC);
-    upload_soa(a.data(), a.size(), d->ptr);
-    return d;
+    return upload_soa(a.data(), a.size(), slot, stream);
 }
 
-inline nd::shared_array<double, 3> down(const device_buffer& d, shape3 s)
+inline double* out(std::size_t slot, std::size_t count) { return mara3p::detail::device_slot(slot, count); }
+
+inline nd::shared_array<double, 3> down(const double* d, shape3 s, std::size_t slot, hipStream_t stream)
 {
     nd::shared_array<double, 3> a(s);
-    hip_check(hipMemcpy(a.data(), d.ptr, count(s) * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    download(d, count(s), a.data(), slot, stream);
     return a;
 }
 
 template<std::size_t NC>
-inline nd::shared_array<std::array<double, NC>, 3> down_tuples(const device_buffer& d, shape3 s)
+inline nd::shared_array<std::array<double, NC>, 3> down_tuples(const double* d, shape3 s, std::size_t slot, hipStream_t stream)
 {
     nd::shared_array<std::array<double, NC>, 3> a(s);
-    download_soa(d.ptr, count(s), a.data());
+    download_soa(d, count(s), a.data(), slot, stream);
     return a;
 }
 
@@ -450,81 +879,99 @@ inline void require(bool ok, const char* what)
 } // namespace detail
 
 // primitive_array, src/scheme_mhd_v2.cpp:75-81
-inline cell_primitive_variables_t primitive_array(cell_conserved_density_t uc, face_magnetic_flux_density_t bf)
+inline cell_primitive_variables_t primitive_array(cell_conserved_density_t uc, face_magnetic_flux_density_t bf,
+                                                  hipStream_t stream = nullptr)
 {
     using namespace detail;
     const shape3 n = uc.shape();
     for (int a = 0; a < 3; ++a) require(bf[a].shape() == face_shape(n, a), "mhd_scheme_v2::primitive_array");
-    auto du = up(uc);
-    auto b1 = up(bf[0]), b2 = up(bf[1]), b3 = up(bf[2]);
-    device_buffer dp(8 * count(n));
-    check(m3p_mhd_v2_primitive_array(du->ptr, b1->ptr, b2->ptr, b3->ptr, int(n[0]), int(n[1]), int(n[2]), dp.ptr, nullptr));
-    return down_tuples<8>(dp, n);
+    double* du = up(uc, 0, stream);
+    double* b1 = up(bf[0], 1, stream);
+    double* b2 = up(bf[1], 2, stream);
+    double* b3 = up(bf[2], 3, stream);
+    double* dp = out(4, 8 * count(n));
+    check(m3p_mhd_v2_primitive_array(du, b1, b2, b3, int(n[0]), int(n[1]), int(n[2]), dp, stream));
+    return down_tuples<8>(dp, n, 4, stream);
 }
 
 // godunov_fluxes, :168-185 (pc extended by one zone on all axes, bf on the transverse axes)
-inline face_godunov_data_t godunov_fluxes(cell_primitive_variables_t pc, face_magnetic_flux_density_t bf)
+inline face_godunov_data_t godunov_fluxes(cell_primitive_variables_t pc, face_magnetic_flux_density_t bf,
+                                          hipStream_t stream = nullptr)
 {
     using namespace detail;
     const shape3 m = pc.shape();
     for (int a = 0; a < 3; ++a)
         require(bf[a].shape() == grow(m, -(a == 0), -(a == 1), -(a == 2)), "mhd_scheme_v2::godunov_fluxes");
-    auto dp = up(pc);
-    auto b1 = up(bf[0]), b2 = up(bf[1]), b3 = up(bf[2]);
-    device_buffer g1(8 * bf[0].size()), g2(8 * bf[1].size()), g3(8 * bf[2].size());
-    check(m3p_mhd_v2_godunov_fluxes(dp->ptr, b1->ptr, b2->ptr, b3->ptr, int(m[0]), int(m[1]), int(m[2]),
-                                    g1.ptr, g2.ptr, g3.ptr, nullptr));
-    return { down_tuples<8>(g1, bf[0].shape()), down_tuples<8>(g2, bf[1].shape()), down_tuples<8>(g3, bf[2].shape()) };
+    double* dp = up(pc, 0, stream);
+    double* b1 = up(bf[0], 1, stream);
+    double* b2 = up(bf[1], 2, stream);
+    double* b3 = up(bf[2], 3, stream);
+    double* g1 = out(4, 8 * bf[0].size());
+    double* g2 = out(5, 8 * bf[1].size());
+    double* g3 = out(6, 8 * bf[2].size());
+    check(m3p_mhd_v2_godunov_fluxes(dp, b1, b2, b3, int(m[0]), int(m[1]), int(m[2]), g1, g2, g3, stream));
+    return { down_tuples<8>(g1, bf[0].shape(), 4, stream), down_tuples<8>(g2, bf[1].shape(), 5, stream),
+             down_tuples<8>(g3, bf[2].shape(), 6, stream) };
 }
 
 // electromotive_forces, :191-214
-inline edge_electromotive_density_t electromotive_forces(face_godunov_data_t gf)
+inline edge_electromotive_density_t electromotive_forces(face_godunov_data_t gf, hipStream_t stream = nullptr)
 {
     using namespace detail;
     const shape3 m = { gf[1].shape(0), gf[0].shape(1), gf[0].shape(2) };
     for (int a = 0; a < 3; ++a)
         require(gf[a].shape() == grow(m, -(a == 0), -(a == 1), -(a == 2)), "mhd_scheme_v2::electromotive_forces");
-    auto g1 = up(gf[0]), g2 = up(gf[1]), g3 = up(gf[2]);
+    double* g1 = up(gf[0], 0, stream);
+    double* g2 = up(gf[1], 1, stream);
+    double* g3 = up(gf[2], 2, stream);
     std::array<shape3, 3> es = { grow(m, 0, -1, -1), grow(m, -1, 0, -1), grow(m, -1, -1, 0) };
-    device_buffer e1(count(es[0])), e2(count(es[1])), e3(count(es[2]));
-    check(m3p_mhd_v2_electromotive_forces(g1->ptr, g2->ptr, g3->ptr, int(m[0]), int(m[1]), int(m[2]),
-                                          e1.ptr, e2.ptr, e3.ptr, nullptr));
-    return { down(e1, es[0]), down(e2, es[1]), down(e3, es[2]) };
+    double* e1 = out(3, count(es[0]));
+    double* e2 = out(4, count(es[1]));
+    double* e3 = out(5, count(es[2]));
+    check(m3p_mhd_v2_electromotive_forces(g1, g2, g3, int(m[0]), int(m[1]), int(m[2]), e1, e2, e3, stream));
+    return { down(e1, es[0], 3, stream), down(e2, es[1], 4, stream), down(e3, es[2], 5, stream) };
 }
 
 // updated_conserved_density, :220-241 (gf extended by one zone on the transverse axes)
 inline cell_conserved_density_t updated_conserved_density(cell_conserved_density_t uc, face_godunov_data_t gf,
-                                                          double dt, double dl)
+                                                          double dt, double dl, hipStream_t stream = nullptr)
 {
     using namespace detail;
     const shape3 n = uc.shape();
     for (int a = 0; a < 3; ++a)
         require(gf[a].shape() == grow(n, a == 0 ? 1 : 2, a == 1 ? 1 : 2, a == 2 ? 1 : 2),
                 "mhd_scheme_v2::updated_conserved_density");
-    auto du = up(uc);
-    auto g1 = up(gf[0]), g2 = up(gf[1]), g3 = up(gf[2]);
-    device_buffer dout(5 * count(n));
-    check(m3p_mhd_v2_updated_conserved_density(du->ptr, g1->ptr, g2->ptr, g3->ptr, int(n[0]), int(n[1]), int(n[2]),
-                                               dt, dl, dout.ptr, nullptr));
-    return down_tuples<5>(dout, n);
+    double* du = up(uc, 0, stream);
+    double* g1 = up(gf[0], 1, stream);
+    double* g2 = up(gf[1], 2, stream);
+    double* g3 = up(gf[2], 3, stream);
+    double* dout = out(4, 5 * count(n));
+    check(m3p_mhd_v2_updated_conserved_density(du, g1, g2, g3, int(n[0]), int(n[1]), int(n[2]), dt, dl, dout, stream));
+    return down_tuples<5>(dout, n, 4, stream);
 }
 
 // curl, :247-258
-inline face_magnetic_flux_density_t curl(edge_electromotive_density_t ee, double dl)
+inline face_magnetic_flux_density_t curl(edge_electromotive_density_t ee, double dl, hipStream_t stream = nullptr)
 {
     using namespace detail;
     const shape3 n = { ee[0].shape(0), ee[1].shape(1), ee[2].shape(2) };
     for (int a = 0; a < 3; ++a) require(ee[a].shape() == edge_shape(n, a), "mhd_scheme_v2::curl");
-    auto e1 = up(ee[0]), e2 = up(ee[1]), e3 = up(ee[2]);
-    device_buffer b1(count(face_shape(n, 0))), b2(count(face_shape(n, 1))), b3(count(face_shape(n, 2)));
-    check(m3p_mhd_v2_curl(e1->ptr, e2->ptr, e3->ptr, int(n[0]), int(n[1]), int(n[2]), dl, b1.ptr, b2.ptr, b3.ptr, nullptr));
-    return { down(b1, face_shape(n, 0)), down(b2, face_shape(n, 1)), down(b3, face_shape(n, 2)) };
+    double* e1 = up(ee[0], 0, stream);
+    double* e2 = up(ee[1], 1, stream);
+    double* e3 = up(ee[2], 2, stream);
+    double* b1 = out(3, count(face_shape(n, 0)));
+    double* b2 = out(4, count(face_shape(n, 1)));
+    double* b3 = out(5, count(face_shape(n, 2)));
+    check(m3p_mhd_v2_curl(e1, e2, e3, int(n[0]), int(n[1]), int(n[2]), dl, b1, b2, b3, stream));
+    return { down(b1, face_shape(n, 0), 3, stream), down(b2, face_shape(n, 1), 4, stream),
+             down(b3, face_shape(n, 2), 5, stream) };
 }
 
 // updated_magnetic_flux_density, :264-278 (ee as electromotive_forces returns it
 // for Godunov data extended by one zone)
 inline face_magnetic_flux_density_t updated_magnetic_flux_density(face_magnetic_flux_density_t bf,
-                                                                  edge_electromotive_density_t ee, double dt, double dl)
+                                                                  edge_electromotive_density_t ee, double dt, double dl,
+                                                                  hipStream_t stream = nullptr)
 {
     using namespace detail;
     const shape3 n = { bf[1].shape(0), bf[0].shape(1), bf[0].shape(2) };
@@ -533,54 +980,60 @@ inline face_magnetic_flux_density_t updated_magnetic_flux_density(face_magnetic_
         require(ee[a].shape() == grow(edge_shape(n, a), 2 * (a == 0), 2 * (a == 1), 2 * (a == 2)),
                 "mhd_scheme_v2::updated_magnetic_flux_density");
     }
-    auto b1 = up(bf[0]), b2 = up(bf[1]), b3 = up(bf[2]);
-    auto e1 = up(ee[0]), e2 = up(ee[1]), e3 = up(ee[2]);
-    device_buffer o1(bf[0].size()), o2(bf[1].size()), o3(bf[2].size());
-    check(m3p_mhd_v2_updated_magnetic_flux_density(b1->ptr, b2->ptr, b3->ptr, e1->ptr, e2->ptr, e3->ptr,
-                                                   int(n[0]), int(n[1]), int(n[2]), dt, dl, o1.ptr, o2.ptr, o3.ptr, nullptr));
-    return { down(o1, bf[0].shape()), down(o2, bf[1].shape()), down(o3, bf[2].shape()) };
+    double* b1 = up(bf[0], 0, stream);
+    double* b2 = up(bf[1], 1, stream);
+    double* b3 = up(bf[2], 2, stream);
+    double* e1 = up(ee[0], 3, stream);
+    double* e2 = up(ee[1], 4, stream);
+    double* e3 = up(ee[2], 5, stream);
+    double* o1 = out(6, bf[0].size());
+    double* o2 = out(7, bf[1].size());
+    double* o3 = out(8, bf[2].size());
+    check(m3p_mhd_v2_updated_magnetic_flux_density(b1, b2, b3, e1, e2, e3, int(n[0]), int(n[1]), int(n[2]), dt, dl,
+                                                   o1, o2, o3, stream));
+    return { down(o1, bf[0].shape(), 6, stream), down(o2, bf[1].shape(), 7, stream), down(o3, bf[2].shape(), 8, stream) };
 }
 
 // extend_cell_primitive_variables, :126-135 (27 blocks in the order of mesh::neighbors_27)
 inline cell_primitive_variables_t extend_cell_primitive_variables(std::vector<cell_primitive_variables_t> pc_vector,
-                                                                  std::size_t block_size, std::size_t count)
+                                                                  std::size_t block_size, std::size_t count,
+                                                                  hipStream_t stream = nullptr)
 {
     using namespace detail;
     if (pc_vector.size() != 27) throw std::invalid_argument("mhd_scheme_v2::extend_cell_primitive_variables (need 27 blocks)");
-    std::vector<std::unique_ptr<device_buffer>> blocks;
     const double* ptrs[27];
     for (std::size_t m = 0; m < 27; ++m) {
         require(pc_vector[m].shape() == shape3{ block_size, block_size, block_size }, "mhd_scheme_v2::extend_cell_primitive_variables");
-        blocks.push_back(up(pc_vector[m]));
-        ptrs[m] = blocks.back()->ptr;
+        ptrs[m] = up(pc_vector[m], m, stream);
     }
     const std::size_t e = block_size + 2 * count;
-    device_buffer out(8 * e * e * e);
-    check(m3p_mhd_v2_extend_cell_primitive_variables(ptrs, int(block_size), int(count), out.ptr, nullptr));
-    return down_tuples<8>(out, { e, e, e });
+    double* o = out(27, 8 * e * e * e);
+    check(m3p_mhd_v2_extend_cell_primitive_variables(ptrs, int(block_size), int(count), o, stream));
+    return down_tuples<8>(o, { e, e, e }, 27, stream);
 }
 
 // extend_face_magnetic_flux_density, :141-162 (27 face triplets, three runs of 9
 // in the order of mesh::neighbors_9 on axes 0, 1, 2)
 inline face_magnetic_flux_density_t extend_face_magnetic_flux_density(std::vector<face_magnetic_flux_density_t> bf_vector,
-                                                                      std::size_t block_size, std::size_t count)
+                                                                      std::size_t block_size, std::size_t count,
+                                                                      hipStream_t stream = nullptr)
 {
     using namespace detail;
     if (bf_vector.size() != 27) throw std::invalid_argument("mhd_scheme_v2::extend_face_magnetic_flux_density (need 27 blocks)");
     const shape3 n = { block_size, block_size, block_size };
-    std::vector<std::unique_ptr<device_buffer>> blocks;
     const double* ptrs[27];
     for (std::size_t m = 0; m < 27; ++m) {
         const int a = int(m / 9);
         require(bf_vector[m][a].shape() == face_shape(n, a), "mhd_scheme_v2::extend_face_magnetic_flux_density");
-        blocks.push_back(up(bf_vector[m][a]));
-        ptrs[m] = blocks.back()->ptr;
+        ptrs[m] = up(bf_vector[m][a], m, stream);
     }
     const std::size_t e = block_size + 2 * count;
     std::array<shape3, 3> os = { shape3{ block_size + 1, e, e }, shape3{ e, block_size + 1, e }, shape3{ e, e, block_size + 1 } };
-    device_buffer o1(detail::count(os[0])), o2(detail::count(os[1])), o3(detail::count(os[2]));
-    check(m3p_mhd_v2_extend_face_magnetic_flux_density(ptrs, int(block_size), int(count), o1.ptr, o2.ptr, o3.ptr, nullptr));
-    return { down(o1, os[0]), down(o2, os[1]), down(o3, os[2]) };
+    double* o1 = out(27, detail::count(os[0]));
+    double* o2 = out(28, detail::count(os[1]));
+    double* o3 = out(29, detail::count(os[2]));
+    check(m3p_mhd_v2_extend_face_magnetic_flux_density(ptrs, int(block_size), int(count), o1, o2, o3, stream));
+    return { down(o1, os[0], 27, stream), down(o2, os[1], 28, stream), down(o3, os[2], 29, stream) };
 }
 } // namespace mhd_scheme_v2
 

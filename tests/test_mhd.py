@@ -2,6 +2,7 @@
 src/scheme_mhd_v1.cpp:265-299): the oracle against the reference's golden
 vectors and known-answer tests, then the HIP kernels against the oracle."""
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -148,3 +149,76 @@ def test_gpu_mhd_div_b_and_mixed():
     assert b.max_div_b() < 1e-12 * scale
     ea, eb = a.state()[0][4], b.state()[0][4]
     assert np.mean(np.abs(ea - eb)) / np.mean(np.abs(ea)) < 1e-4
+
+
+@pytest.mark.gpu
+@gpu
+@pytest.mark.parametrize("N", [12, 24, 32, 72])
+def test_gpu_mhd_mixed_tracks_fp64(N):
+    """Every mixed-precision MHD code path against the FP64 kernel, all four
+    output arrays: N % 8 != 0 runs the per-cell passes (N = 12), the 8 x 8
+    column tiles (N = 24), the 8 x 32 column tiles in one segment (N = 32) and
+    in 8-plane segments (N = 72, N % 16 == 8).  Three RK2 steps from a
+    perturbed Orszag-Tang state; each kernel takes its own CFL time step, so
+    the bar (1e-5 of each array's max) covers FP32 fluxes and FP32 speeds."""
+    from mara3p_amd.problems import orszag_tang_3d
+    from mara3p_amd.solvers import MhdSolver
+    uc, b1, b2, b3 = orszag_tang_3d(N, seed=5, perturbation=0.05)
+    a, b = MhdSolver(N), MhdSolver(N, precision="mixed")
+    a.set_state(uc, b1, b2, b3)
+    b.set_state(uc, b1, b2, b3)
+    for _ in range(3):
+        a.step()
+        b.step()
+    a.check_errors()
+    b.check_errors()
+    ga, gb = a.state(), b.state()
+    for q in range(5):
+        err = _rel(gb[0][q], ga[0][q])
+        assert err <= 1e-5, (N, "U", q, err)
+    for ax in range(1, 4):
+        err = _rel(gb[ax], ga[ax])
+        assert err <= 1e-5, (N, "B", ax, err)
+    scale = np.abs(b1).max() / (1.0 / N)
+    assert b.max_div_b() < 1e-12 * scale
+
+
+@pytest.mark.gpu
+@gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_mhd_256_cubed_properties(precision):
+    """The 3D MHD configuration at its full size, 256^3, by properties the
+    oracle (serial) cannot reach there: from the z-independent Orszag-Tang
+    state, two RK2 steps keep (1) max |div B| at round-off, (2) every z plane
+    identical, (3) the point symmetry (x, y) -> (1 - x, 1 - y) of the initial
+    state (density and energy even, momenta and fields odd), to 1e-13 of each
+    array's max in FP64 and 1e-6 in the mixed kernel, and (4) the totals of
+    mass, momentum and energy to round-off (periodic cube)."""
+    from mara3p_amd.problems import orszag_tang_3d
+    from mara3p_amd.solvers import MhdSolver
+    N = 256
+    uc, b1, b2, b3 = orszag_tang_3d(N, seed=1, perturbation=0.0)
+    s = MhdSolver(N, precision=precision)
+    s.set_state(uc, b1, b2, b3)
+    tot0 = [math.fsum(uc[q].ravel()) for q in range(5)]
+    for _ in range(2):
+        s.step()
+    s.check_errors()
+    scale = np.abs(b1).max() / (1.0 / N)
+    assert s.max_div_b() < 1e-12 * scale
+    u, f1, f2, f3 = s.state()
+    assert np.array_equal(u, np.broadcast_to(u[:, :, :, :1], u.shape))
+    assert np.abs(f3).max() == 0.0
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    # reversing an axis maps cells i -> N - 1 - i and faces i -> N - i
+    for q, sign in ((0, 1), (1, -1), (2, -1), (4, 1)):
+        a = u[q, :, :, 0]
+        assert _rel(sign * a[::-1, ::-1], a) <= tol, ("U", q)
+    assert _rel(-f1[::-1, ::-1, 0], f1[:, :, 0]) <= tol
+    assert _rel(-f2[::-1, ::-1, 0], f2[:, :, 0]) <= tol
+    for q in (0, 4):
+        t = math.fsum(u[q].ravel())
+        assert abs(t - tot0[q]) <= 1e-13 * abs(tot0[q]), ("total", q)
+    for q in (1, 2):
+        t = math.fsum(u[q].ravel())
+        assert abs(t - tot0[q]) <= 1e-13 * math.fsum(np.abs(uc[q]).ravel()), ("total", q)

@@ -427,3 +427,67 @@ def test_sod_converges_to_exact_solution():
         ex[i] = exact[0]
     l1 = np.mean(np.abs(rho - ex))
     assert l1 < 3e-3, l1
+
+
+# Toro, Riemann Solvers and Numerical Methods for Fluid Dynamics, Table 4.3:
+# exact star states of the five tests of Table 4.1 (gamma = 1.4):
+# (left rho, u, p), (right rho, u, p), p*, u*, rho*L, rho*R, as printed.
+TORO_STAR = [
+    ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.30313, 0.92745, 0.42632, 0.26557),
+    ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), 0.00189, 0.00000, 0.02185, 0.02185),
+    ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), 460.894, 19.5975, 0.57506, 5.99924),
+    ((1.0, 0.0, 0.01), (1.0, 0.0, 100.0), 46.0950, -6.19633, 5.99242, 0.57511),
+    ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), 1691.64, 8.68975, 14.2823, 31.0426),
+]
+
+
+def _printed_tol(v):
+    """One unit in the last printed digit of Toro's value (six significant
+    figures, or five decimals for values below one; the table rounds or
+    truncates, and test 5 starts from test 3's rounded star states)."""
+    v = abs(v)
+    if v < 1.0:
+        return 1e-5
+    digits = 6 - (int(np.floor(np.log10(v))) + 1)
+    return 10.0 ** (-digits)
+
+
+@pytest.mark.parametrize("case", range(len(TORO_STAR)))
+def test_exact_riemann_matches_toro_star_states(case):
+    """The exact Riemann solver that the Sod checks use as their yardstick,
+    pinned against Toro's tabulated star states: the sample just left of the
+    contact (x/t = u* - 1e-3) gives p*, u* and rho*L, just right rho*R (the
+    star region is wider than that in every test)."""
+    left, right, p_star, u_star, rho_l, rho_r = TORO_STAR[case]
+    if case == 4:
+        # test 5 starts from test 3's star states rounded to six figures,
+        # which moves its exact star state by a few parts in 1e6
+        tol = lambda v: max(_printed_tol(v), 1e-5 * abs(v))  # noqa: E731
+    else:
+        tol = _printed_tol
+    out_l, out_r = np.zeros(3), np.zeros(3)
+    orc().orc_euler_exact_riemann(P(c(left)), P(c(right)), 1.4, u_star - 1e-3, P(out_l))
+    orc().orc_euler_exact_riemann(P(c(left)), P(c(right)), 1.4, u_star + 1e-3, P(out_r))
+    assert abs(out_l[2] - p_star) <= tol(p_star), (out_l[2], p_star)
+    assert abs(out_l[1] - u_star) <= tol(u_star), (out_l[1], u_star)
+    assert abs(out_l[0] - rho_l) <= tol(rho_l), (out_l[0], rho_l)
+    assert abs(out_r[0] - rho_r) <= tol(rho_r), (out_r[0], rho_r)
+    assert abs(out_r[2] - p_star) <= tol(p_star)
+
+
+def test_exact_riemann_far_field_and_sod_fan():
+    """Outside the wave fan the sample is the initial state; inside the Sod
+    rarefaction (Toro test 1, x/t between -c_L and u* - c*_L) it follows the
+    closed form rho = rho_L (2/(g+1) + (g-1)/((g+1) c_L) (u_L - x/t))^(2/(g-1))."""
+    g = 1.4
+    left, right = c([1.0, 0.0, 1.0]), c([0.125, 0.0, 0.1])
+    out = np.zeros(3)
+    orc().orc_euler_exact_riemann(P(left), P(right), g, -5.0, P(out))
+    assert np.array_equal(out, left)
+    orc().orc_euler_exact_riemann(P(left), P(right), g, 5.0, P(out))
+    assert np.array_equal(out, right)
+    cl = np.sqrt(g * 1.0 / 1.0)
+    for xi in (-1.1, -0.9, -0.5, -0.2):
+        orc().orc_euler_exact_riemann(P(left), P(right), g, xi, P(out))
+        rho = (2.0 / (g + 1) + (g - 1) / ((g + 1) * cl) * (0.0 - xi)) ** (2.0 / (g - 1))
+        assert abs(out[0] - rho) <= 1e-12, (xi, out[0], rho)
