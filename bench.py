@@ -248,67 +248,73 @@ def host_cpu():
     return {"model": model, "nproc": os.cpu_count()}
 
 
-def cpu_baseline(workload, u0, dt):
-    """Reference CPU path on the host cores: the reference's own
-    minidisk::updated_conserved (oracle/_ref) when built, else the oracle,
-    one task per block per stage on a thread pool (the reference's scheduling,
-    src/parallel_computable.cpp:248-296).  Sample: one RK2 step of the mesh."""
-    from concurrent.futures import ThreadPoolExecutor
+def cpu_baseline(workload, u0, dt, min_seconds=3.0):
+    """Reference CPU path on the host cores, timed on a bounded sample.
+
+    minidisk: the reference's own functions compiled from its sources
+    (oracle/_ref, -O3, x86-64-v3), the mesh held in the reference's array
+    types, one RK2 step = per block minidisk::recover_primitive_array, then
+    per block the guard fill and minidisk::updated_conserved, twice, and the
+    weighted sum, every per-block task a job of the reference's
+    mara::ThreadPool (src/parallel_thread_pool.hpp, the scheduling of
+    src/parallel_computable.cpp:248-296) with one worker per host core; the
+    calling thread only waits.  One RK2 step of warm-up, then whole RK2 steps
+    until at least `min_seconds` have passed.  Without the reference build
+    (and for kh) the oracle's C restatement with OpenMP over blocks."""
     import oracle
     from oracle import OrcMinidisk, P, orc
     from mara3p_amd import MinidiskData
     from mara3p_amd.mesh import to_reference_layout
-    threads = min(os.cpu_count() or 1, 16)
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    u = np.ascontiguousarray(to_reference_layout(u0))
+    nblk = u.shape[0]
+    zones = nblk * u.shape[1] * u.shape[2]
+
+    def timed(step):
+        step()                               # warm-up
+        n, t0 = 0, time.perf_counter()
+        while True:
+            step()
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= min_seconds:
+                return n, el
+
     if workload != "minidisk":
-        u = np.ascontiguousarray(to_reference_layout(u0))
-        nblk = u.shape[0]
-        t0 = time.perf_counter()
-        orc().orc_euler2d_mesh_rk2(P(u), 16, 256, dt, 1.0 / 4096, 1.5, 5.0 / 3.0, 1, 1, threads)
-        el = time.perf_counter() - t0
-        zones = nblk * 256 * 256
-        return {"value": zones / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-                "sample": "one RK2 step of the full 4096^2 mesh, oracle (C restatement, -O2), OpenMP over blocks"}
+        state = u.copy()
+        steps, el = timed(lambda: orc().orc_euler2d_mesh_rk2(P(state), 16, 256, dt, 1.0 / 4096, 1.5, 5.0 / 3.0,
+                                                            1, 1, threads))
+        return {"value": zones * steps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+                "sample": "%d RK2 steps of the full 4096^2 mesh after one of warm-up (%.1f s): oracle C "
+                          "restatement (-O2), OpenMP over blocks" % (steps, el)}
     data = MinidiskData.defaults(block_size=256, depth=4, domain_radius=8.0, softening_length=0.05, sink_radius=0.05)
     sd = OrcMinidisk.from_data(data)
-    sda = sd.as_array()
-    u = np.ascontiguousarray(to_reference_layout(u0))
-    nb, n = 16, 256
-    use_ref = oracle.ref_available()
-    kind = "reference" if use_ref else "port"
+    if oracle.ref_available():
+        h = oracle.ref().ref_minidisk_mesh_create(P(u), P(sd.as_array()), threads)
+        try:
+            t = [0.0]
 
-    def stage(src, time_):
-        P_ = np.empty_like(src)
-        P_[..., 0] = src[..., 0]
-        P_[..., 1] = src[..., 1] / src[..., 0]
-        P_[..., 2] = src[..., 2] / src[..., 0]
-        P_ = np.ascontiguousarray(P_)
-        out = np.empty_like(src)
+            def step():
+                oracle.ref().ref_minidisk_mesh_rk2(h, t[0], dt, 1)
+                t[0] += dt
+            steps, el = timed(step)
+        finally:
+            oracle.ref().ref_minidisk_mesh_destroy(h)
+        return {"value": zones * steps / el, "unit": "zone-updates/s", "cores": threads, "kind": "reference",
+                "sample": "%d RK2 steps of the full 16x16x256^2 mesh after one of warm-up (%.1f s): the reference's "
+                          "recover_primitive_array, guard fill and minidisk::updated_conserved compiled from its "
+                          "sources (-O3, x86-64-v3), one mara::ThreadPool task per block per pass, %d workers, "
+                          "calling thread waiting" % (steps, el, threads)}
+    state = u.copy()
+    t = [0.0]
 
-        def task(b):
-            pe = np.empty((n + 4, n + 4, 3))
-            orc().orc_extend_block_2d(P(P_), nb, n, 3, b // nb, b % nb, P(pe))
-            o = np.empty((n, n, 3))
-            if use_ref:
-                oracle.ref().ref_minidisk_updated_conserved(P(np.ascontiguousarray(src[b])), P(pe), P(sda), 4,
-                                                            b // nb, b % nb, time_, dt, P(o))
-            else:
-                orc().orc_minidisk_updated_conserved(P(np.ascontiguousarray(src[b])), P(pe), ctypes.byref(sd), 4,
-                                                     b // nb, b % nb, time_, dt, P(o))
-            out[b] = o
-        with ThreadPoolExecutor(threads) as ex:
-            list(ex.map(task, range(nb * nb)))
-        return out
-
-    t0 = time.perf_counter()
-    u1 = stage(u, 0.0)
-    u2 = stage(u1, dt)
-    _ = u * 0.5 + u2 * 0.5
-    el = time.perf_counter() - t0
-    zones = nb * nb * n * n
-    what = ("reference minidisk::updated_conserved compiled from its sources (-O2)" if use_ref
-            else "oracle C restatement (-O2)")
-    return {"value": zones / el, "unit": "zone-updates/s", "cores": threads, "kind": kind,
-            "sample": "one RK2 step of the full 16x16x256^2 mesh: %s, one task per block per stage" % what}
+    def step():
+        orc().orc_minidisk_mesh_rk2(P(state), sd, t[0], dt, 1, threads)
+        t[0] += dt
+    steps, el = timed(step)
+    return {"value": zones * steps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": "%d RK2 steps of the full 16x16x256^2 mesh after one of warm-up (%.1f s): oracle C "
+                      "restatement (-O2), OpenMP over blocks" % (steps, el)}
 
 
 def traffic_from_profiles(workload, precision):

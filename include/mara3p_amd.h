@@ -187,6 +187,24 @@ int  m3p_mhd_advance_ct(const double* uc, const double* bf1, const double* bf2, 
                         int N, double dl, double cfl_number, double plm_theta, double gamma_law_index,
                         double* uo, double* bo1, double* bo2, double* bo3, double* dt,
                         void* work, long work_bytes, int precision, void* stream);
+/* m3p_mhd_advance_ct for callers that cannot allocate (no reference
+ * counterpart: for code under hipGraph capture, tight serving loops and
+ * harnesses that forbid hipMalloc).  The same kernels and results as
+ * m3p_mhd_advance_ct, with no workspace argument: the workspace is a static
+ * device array of 48 MiB inside the library (one per GPU, held until the
+ * process ends), so N is limited to what fits (m3p_mhd_workspace_bytes(N,
+ * precision) <= 48 MiB: N <= 59 in FP64, <= 75 mixed); a larger N fails with
+ * M3P_ERR_BAD_ARGUMENT.  Kernel errors (negative density or pressure) go to
+ * status, a caller's device int[4] (code, block, zone) that the call clears
+ * first; m3p_check_errors does not see them.  Nothing is allocated, with
+ * hipMalloc, torch or otherwise.  All calls on one GPU share the one array,
+ * which keeps the last call's contents: calls that overlap on one GPU (other
+ * streams or threads) race and must be serialised by the caller. */
+int  m3p_mhd_advance_ct_noalloc(const double* uc, const double* bf1, const double* bf2, const double* bf3,
+                                const double* u0, const double* b01, const double* b02, const double* b03, double rk_b,
+                                int N, double dl, double cfl_number, double plm_theta, double gamma_law_index,
+                                double* uo, double* bo1, double* bo2, double* bo3, double* dt, int* status,
+                                int precision, void* stream);
 /* max over cells of |div B| = |d1 B1 + d2 B2 + d3 B3| (divergence_difference,
  * src/mesh_cartesian_3d.hpp:290-301) into out (device double). */
 int  m3p_mhd_max_div_b(const double* bf1, const double* bf2, const double* bf3, int N, double* out, void* stream);

@@ -113,6 +113,8 @@ SIGNATURES = {
     "m3p_mhd_workspace_bytes": (_L, [_I, _I]),
     "m3p_mhd_advance_ct": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P,
                                 _P, _L, _I, _P]),
+    "m3p_mhd_advance_ct_noalloc": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P,
+                                        _P, _I, _P]),
     "m3p_mhd_max_div_b": (_I, [_P, _P, _P, _I, _P, _P]),
     "m3p_mhd_v2_primitive_array": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P]),
     "m3p_mhd_v2_godunov_fluxes": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),

@@ -375,6 +375,40 @@ totals_final_kernel(const double* partial, int nparts, double* totals)
 
 } // namespace
 
+// ---------------------------------------------------------------------------
+// Static scratch of m3p_mhd_advance_ct_noalloc: the MHD workspace for callers
+// that cannot allocate.  One copy per GPU, held until the process ends.  Host
+// code never passes the array's name to a kernel (on the host it names a
+// stand-in, not device memory): each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr long kScratchBytes = 48l << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void** out)
+{
+    int dev;
+    if (int st = current_device(&dev)) return st;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (int st = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return st;
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return M3P_OK;
+}
+
+__global__ void clear_status_kernel(int* status)
+{
+    if (threadIdx.x < 4) status[threadIdx.x] = 0;
+}
+
+} // namespace
+
 // ===========================================================================
 extern "C" {
 
@@ -638,6 +672,41 @@ int m3p_mhd_advance_ct(const double* uc, const double* bf1, const double* bf2, c
     a.N = N; a.dl = dl; a.cfl = cfl_number; a.theta = plm_theta; a.gamma = gamma_law_index;
     a.dt = dt; a.err = ds->err;
     hipStream_t s = (hipStream_t)stream;
+    if (precision == M3P_FP64) return mhd_advance<double>(a, (char*)work, s);
+    return mhd_advance<float>(a, (char*)work, s);
+}
+
+int m3p_mhd_advance_ct_noalloc(const double* uc, const double* bf1, const double* bf2, const double* bf3,
+                               const double* u0, const double* b01, const double* b02, const double* b03, double rk_b,
+                               int N, double dl, double cfl_number, double plm_theta, double gamma_law_index,
+                               double* uo, double* bo1, double* bo2, double* bo3, double* dt, int* status,
+                               int precision, void* stream)
+{
+    const char* what = "m3p_mhd_advance_ct_noalloc";
+    if (!uc || !bf1 || !bf2 || !bf3 || !uo || !bo1 || !bo2 || !bo3 || !status || N < 2)
+        return fail(M3P_ERR_BAD_ARGUMENT, std::string(what) + ": bad arguments");
+    if (u0 && (!b01 || !b02 || !b03)) return fail(M3P_ERR_BAD_ARGUMENT, std::string(what) + ": RK base needs all four arrays");
+    if (uo == uc || bo1 == bf1 || bo2 == bf2 || bo3 == bf3)
+        return fail(M3P_ERR_BAD_ARGUMENT, std::string(what) + ": outputs must not alias inputs");
+    const long need = mhd_workspace_bytes(N, precision);
+    if (need > kScratchBytes) {
+        int nmax = N;
+        while (nmax > 2 && mhd_workspace_bytes(nmax, precision) > kScratchBytes) --nmax;
+        return fail(M3P_ERR_BAD_ARGUMENT, std::string(what) + ": N = " + std::to_string(N) + " needs " +
+                    std::to_string(need) + " bytes of workspace, more than the " + std::to_string(kScratchBytes) +
+                    "-byte static scratch (largest N in this precision: " + std::to_string(nmax) + ")");
+    }
+    void* work = nullptr;
+    if (int st = scratch_address(&work)) return st;
+    MhdArgs a{};
+    a.uc = uc; a.b1 = bf1; a.b2 = bf2; a.b3 = bf3;
+    a.u0 = u0; a.b01 = b01; a.b02 = b02; a.b03 = b03; a.rk_b = rk_b;
+    a.uo = uo; a.bo1 = bo1; a.bo2 = bo2; a.bo3 = bo3;
+    a.N = N; a.dl = dl; a.cfl = cfl_number; a.theta = plm_theta; a.gamma = gamma_law_index;
+    a.dt = dt; a.err = status;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(clear_status_kernel, dim3(1), dim3(64), 0, s, status);
+    if (int st = launch_status("clear_status_kernel")) return st;
     if (precision == M3P_FP64) return mhd_advance<double>(a, (char*)work, s);
     return mhd_advance<float>(a, (char*)work, s);
 }

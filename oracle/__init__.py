@@ -93,6 +93,10 @@ _REF_SIG = {
     "ref_two_body_recover_elements": (None, [_P, _D, _P]),
     "ref_minidisk_updated_conserved": (None, [_P, _P, _P, _I, _L, _L, _D, _D, _P]),
     "ref_minidisk_initial_conserved": (None, [_P, _I, _L, _L, _P]),
+    "ref_minidisk_mesh_create": (_P, [_P, _P, _I]),
+    "ref_minidisk_mesh_rk2": (None, [_P, _D, _D, _I]),
+    "ref_minidisk_mesh_get": (None, [_P, _P]),
+    "ref_minidisk_mesh_destroy": (None, [_P]),
     "ref_mhd_recover_primitive": (_I, [_P, _P, _D, _P]),
     "ref_mhd_conserved_density": (None, [_P, _D, _P]),
     "ref_mhd_flux": (None, [_P, _I, _D, _P]),

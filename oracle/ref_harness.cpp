@@ -21,6 +21,7 @@
 #include "mesh_block_index.hpp"
 #include "minidisk.hpp"
 #include "scheme_mhd_v1.hpp"
+#include "parallel_thread_pool.hpp"
 
 namespace {
 
@@ -172,6 +173,118 @@ void ref_minidisk_initial_conserved(const double* sd, int level, long bi, long b
         for (unsigned long j = 0; j < n; ++j)
             to_array(u(i, j), &out[(i * n + j) * 3]);
 }
+
+// ---------------------------------------------------------------------------
+// A uniform periodic minidisk mesh held in the reference's own types, stepped
+// by the reference's functions on the reference's thread pool (the CPU
+// baseline).  One stage has the task structure of the reference's substep
+// (problems/minidisk.cpp:172-189): per block "P" =
+// minidisk::recover_primitive_array, then per block "Pe" (the extended
+// primitive array from the nine neighbour blocks, problems/minidisk.cpp:87-120)
+// followed by "U" = minidisk::updated_conserved.  RK2 combines per block as
+// amr::weighted_sum_tree (src/mesh_amr.hpp:455-467).  Every task is one
+// mara::ThreadPool job (src/parallel_thread_pool.hpp:121-137); the calling
+// thread waits on the futures.
+struct ref_minidisk_mesh {
+    minidisk::solver_data_t data;
+    unsigned long nb = 0;
+    std::vector<minidisk::conserved_array_t> u;
+    std::unique_ptr<mara::ThreadPool> pool;
+};
+
+namespace {
+
+minidisk::primitive_array_t extend_from(const std::vector<minidisk::primitive_array_t>& P, unsigned long nb,
+                                        unsigned long bi, unsigned long bj)
+{
+    auto wrap = [nb] (unsigned long c, int d) { return (c + nb + d) % nb; };
+    // the corner blocks as problems/minidisk.cpp:90-93 picks them
+    std::array<std::array<minidisk::primitive_array_t, 3>, 3> cs;
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) cs[a][c] = P[wrap(bi, a - 1) * nb + wrap(bj, c - 1)];
+    cs[0][2] = P[wrap(bi, 1) * nb + wrap(bj, -1)];
+    cs[2][0] = P[wrap(bi, -1) * nb + wrap(bj, 1)];
+    const auto nx = shape(cs[1][1], 0), ny = shape(cs[1][1], 1);
+    return nd::make_array(nd::indexing([cs, nx, ny] (auto i, auto j)
+    {
+        auto ci = i < 2 ? 0 : (i >= nx + 2 ? 2 : 1);
+        auto cj = j < 2 ? 0 : (j >= ny + 2 ? 2 : 1);
+        auto ii = i < 2 ? i - 2 + nx : (i >= nx + 2 ? i - 2 - nx : i - 2);
+        auto jj = j < 2 ? j - 2 + ny : (j >= ny + 2 ? j - 2 - ny : j - 2);
+        return cs[ci][cj](ii, jj);
+    }), nd::uivec(nx + 4, ny + 4)) | nd::to_shared();
+}
+
+std::vector<minidisk::conserved_array_t> mesh_stage(ref_minidisk_mesh& m, const std::vector<minidisk::conserved_array_t>& u,
+                                                    double time, double dt)
+{
+    const unsigned long nb = m.nb, nblk = nb * nb;
+    std::vector<std::future<minidisk::primitive_array_t>> fp;
+    for (unsigned long b = 0; b < nblk; ++b)
+        fp.push_back(m.pool->enqueue(0, [&u, b] { return minidisk::recover_primitive_array(u[b]); }));
+    std::vector<minidisk::primitive_array_t> P;
+    for (auto& f : fp) P.push_back(f.get());
+    std::vector<std::future<minidisk::conserved_array_t>> fu;
+    for (unsigned long b = 0; b < nblk; ++b)
+        fu.push_back(m.pool->enqueue(0, [&m, &u, &P, b, nb, time, dt] {
+            auto pe = extend_from(P, nb, b / nb, b % nb);
+            auto block = mesh::block_index_t<2>{ (unsigned long)m.data.depth, numeric::array(b / nb, b % nb) };
+            return minidisk::updated_conserved(u[b], pe, time, dt, block, m.data);
+        }));
+    std::vector<minidisk::conserved_array_t> out;
+    for (auto& f : fu) out.push_back(f.get());
+    return out;
+}
+
+} // namespace
+
+extern "C" {
+
+// u: [nb*nb][n][n][3] conserved, block b = bi * nb + bj at the solver's depth.
+ref_minidisk_mesh* ref_minidisk_mesh_create(const double* u, const double* sd, int nthreads)
+{
+    auto m = new ref_minidisk_mesh;
+    m->data = solver_data(sd);
+    m->nb = 1ul << m->data.depth;
+    const unsigned long n = m->data.block_size;
+    for (unsigned long b = 0; b < m->nb * m->nb; ++b) {
+        const double* ub = u + b * n * n * 3;
+        m->u.push_back(nd::make_array(nd::indexing([ub, n] (auto i, auto j)
+        {
+            return iso_cons(&ub[(i * n + j) * 3]);
+        }), nd::uivec(n, n)) | nd::to_shared());
+    }
+    m->pool = std::make_unique<mara::ThreadPool>(nthreads);
+    return m;
+}
+
+// RK2 steps as control::advance_runge_kutta (src/app_control.hpp:169-195).
+void ref_minidisk_mesh_rk2(ref_minidisk_mesh* m, double time, double dt, int steps)
+{
+    for (int s = 0; s < steps; ++s) {
+        auto u1 = mesh_stage(*m, m->u, time, dt);
+        auto u2 = mesh_stage(*m, u1, time + dt, dt);
+        std::vector<std::future<minidisk::conserved_array_t>> fw;
+        for (unsigned long b = 0; b < m->u.size(); ++b)
+            fw.push_back(m->pool->enqueue(0, [m, &u2, b] {
+                return (m->u[b] * 0.5 + u2[b] * (1.0 - 0.5)) | nd::to_shared();
+            }));
+        for (unsigned long b = 0; b < m->u.size(); ++b) m->u[b] = fw[b].get();
+        time = time * 0.5 + (time + dt) * (1.0 - 0.5);
+    }
+}
+
+void ref_minidisk_mesh_get(const ref_minidisk_mesh* m, double* out)
+{
+    const unsigned long n = m->data.block_size;
+    for (unsigned long b = 0; b < m->u.size(); ++b)
+        for (unsigned long i = 0; i < n; ++i)
+            for (unsigned long j = 0; j < n; ++j) to_array(m->u[b](i, j), &out[((b * n + i) * n + j) * 3]);
+}
+
+void ref_minidisk_mesh_destroy(ref_minidisk_mesh* m) { delete m; }
+
+} // extern "C"
 
 // ---------------------------------------------------------------------------
 // MHD: src/physics_mhd.hpp and mara::advance_mhd_ct (src/scheme_mhd_v1.cpp:265-299)

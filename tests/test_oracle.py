@@ -168,6 +168,31 @@ def test_minidisk_live_reference_random_blocks():
 
 
 @needs_ref
+def test_reference_mesh_session_matches_oracle_mesh_rk2():
+    """The CPU baseline's reference path (the reference's recover, guard fill
+    and minidisk::updated_conserved on its own thread pool, RK2 by
+    weighted_sum) equals the oracle's mesh RK2 bit for bit on a 4 x 4 mesh of
+    16^2 blocks over two steps."""
+    from mara3p_amd import MinidiskData, problems
+    from mara3p_amd.mesh import to_reference_layout
+    data = MinidiskData.defaults(block_size=16, depth=2, domain_radius=2.0, softening_length=0.05, sink_radius=0.05)
+    _, u0 = problems.minidisk(data, seed=3)
+    u = c(to_reference_layout(u0))
+    sd = OrcMinidisk.from_data(data)
+    dt = problems.minidisk_dt(data, 0.0)
+    h = oracle.ref().ref_minidisk_mesh_create(P(u), P(sd.as_array()), 3)
+    try:
+        oracle.ref().ref_minidisk_mesh_rk2(h, 0.0, dt, 2)
+        got = np.zeros_like(u)
+        oracle.ref().ref_minidisk_mesh_get(h, P(got))
+    finally:
+        oracle.ref().ref_minidisk_mesh_destroy(h)
+    ref = u.copy()
+    assert orc().orc_minidisk_mesh_rk2(P(ref), sd, 0.0, dt, 2, 2) == 0
+    assert np.array_equal(got, ref)
+
+
+@needs_ref
 def test_pointwise_live_reference():
     rng = np.random.default_rng(11)
     for _ in range(2000):
