@@ -318,6 +318,20 @@ int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, con
 int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
                           const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
                           double plm_theta, double gamma_law_index, int hllc, int precision, void* stream);
+/* The quad-tree stage above in a conservative mode (no reference
+ * counterpart: the reference applies no flux correction at level jumps,
+ * src/mesh_amr.hpp:300-328, problems/minidisk.cpp:60-62).  Fused guard fill
+ * as m3p_amr_euler2d_stage with pe_work NULL; the stage also writes the face
+ * fluxes of every leaf edge to flux_work [nleaf][4][5][n] (device, caller's;
+ * edges -x, +x, -y, +y), then every zone next to an edge bordering finer
+ * leaves is corrected so that its flux through that edge is the mean of the
+ * two finer leaves' face fluxes covering each face (the mass, momentum and
+ * energy leaving one side enter the other, to round-off).  Zones away from
+ * such edges are unchanged. */
+int m3p_amr_euler2d_stage_conservative(const double* u, const double* u0, double* out, double* flux_work,
+                                       const int* recipes, const double* dl_leaf, int nleaf, int n, double dt,
+                                       const double* dt_dev, double rk_b, double plm_theta, double gamma_law_index,
+                                       int hllc, int precision, void* stream);
 /* CFL with a cell spacing per block: dt = cfl / max over zones of
  * (max(|v1|, |v2|) + c_s) / dl_b (the reference fixes dt, src/mara.cpp:305-308;
  * this time step is this library's). */

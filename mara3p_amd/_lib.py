@@ -127,6 +127,7 @@ SIGNATURES = {
     "m3p_srhd_recover_primitive_array": (_I, [_P, _P, _L, _D, _D, _P]),
     "m3p_euler1d_stage_blocks": (_I, [_P, _P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _I, _P]),
     "m3p_amr_euler2d_stage": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _I, _I, _P]),
+    "m3p_amr_euler2d_stage_conservative": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _I, _I, _P]),
     "m3p_euler_cfl_dt_blocks": (_I, [_P, _I, _L, _P, _D, _D, _P, _P]),
     "m3p_euler2d_stage_cfl": (_I, [_P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _D, _I, _D, _P, _I, _P]),
     "m3p_euler1d_stage_blocks_cfl": (_I, [_P, _P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _D, _P, _I, _P]),

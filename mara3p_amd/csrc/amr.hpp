@@ -85,4 +85,52 @@ amr_guard_fill_kernel(const double* u, double* pe, const int* recipes, int nleaf
     if (!ok) report_error(err, 1, b, (i * n + j));
 }
 
+// Flux correction at level jumps (the conservative quad-tree mode; the
+// reference has none, src/mesh_amr.hpp:300-328, problems/minidisk.cpp:60-62).
+// The stage wrote every leaf's edge face fluxes into reg [nleaf][4][5][n]
+// (edges -x, +x, -y, +y).  Where a leaf's edge borders two finer leaves, the
+// flux through each of its edge faces becomes the mean of the two fine face
+// fluxes covering it (the fine side's fluxes, per unit length), so what
+// leaves one side enters the other: the zone next to the edge changes by
+// s dt / dl (F_fine_mean - F_own), s = +1 on the lower edges (-x, -y) and -1
+// on the upper ones, times w = 1 - rk_b after an RK combine.  One launch per
+// axis (phase 0: x edges, rows 0 and n-1; phase 1: y edges, columns 0 and
+// n-1), so a corner zone takes its two corrections in a fixed order.
+__global__ void __launch_bounds__(256)
+amr_reflux_kernel(double* out, const double* reg, const int* recipes, const double* dl_leaf, int nleaf, int n,
+                  double dt, const double* dt_dev, double w, int phase)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nleaf * 2 * n) return;
+    const int b = (int)(idx / (2 * n));
+    const int e = 2 * phase + (int)(idx / n % 2);       // edge
+    const int i = (int)(idx % n);                       // position along the edge
+    const int* rec = recipes + ((size_t)b * 4 + e) * 8;
+    if (rec[0] != 2) return;
+    // the two finer leaves' faces covering face i: children on the side
+    // facing this leaf (recipe children in the order (0,0), (1,0), (0,1),
+    // (1,1) of their (x, y) position inside the neighbour block)
+    const int half = (2 * i) / n, fi = 2 * i - half * n;
+    int child, fe;
+    if (e == 0)      { child = rec[2 + 1 + 2 * half]; fe = 1; }
+    else if (e == 1) { child = rec[2 + 0 + 2 * half]; fe = 0; }
+    else if (e == 2) { child = rec[2 + half + 2];     fe = 3; }
+    else             { child = rec[2 + half];         fe = 2; }
+    const int r = e == 0 ? 0 : (e == 1 ? n - 1 : i);
+    const int c = e == 2 ? 0 : (e == 3 ? n - 1 : i);
+    const double step = dt_dev ? *dt_dev : dt;
+    const double dl = dl_leaf[b];
+    const double s = (e == 0 || e == 2) ? 1.0 : -1.0;
+    const size_t zpb = (size_t)n * n;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double* fr = reg + ((size_t)child * 4 + fe) * 5 * n + (size_t)m * n;
+        const double fmean = 0.5 * (fr[fi] + fr[fi + 1]);
+        const double own = reg[(((size_t)b * 4 + e) * 5 + m) * n + i];
+        const double corr = s * ((fmean - own) * step / dl);
+        double* z = out + ((size_t)b * 5 + m) * zpb + (size_t)r * n + c;
+        *z = *z + corr * w;
+    }
+}
+
 } // namespace m3p

@@ -795,7 +795,7 @@ int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, con
 static int amr_euler2d_stage(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
                              const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
                              double plm_theta, double gamma_law_index, int hllc, int precision, void* stream,
-                             unsigned long long* rate_slot)
+                             unsigned long long* rate_slot, double* flux_reg = nullptr)
 {
     StreamState* ds;
     if (int st = stream_state(stream, &ds)) return st;
@@ -806,12 +806,19 @@ static int amr_euler2d_stage(const double* u, const double* u0, double* out, dou
     hipStream_t s = (hipStream_t)stream;
     StageArgs a{};
     a.u0 = u0; a.out = out; a.dt_dev = dt_dev; a.dt = dt; a.dl_blk = dl_leaf; a.rk_b = rk_b;
-    a.n = n; a.nblk = nleaf; a.err = ds->err; a.rate_slot = rate_slot;
+    a.n = n; a.nblk = nleaf; a.err = ds->err; a.rate_slot = rate_slot; a.flux_reg = flux_reg;
     if (!pe_work) {
         // fused: guard zones read through the recipes inside the stage kernel
-        if (precision == M3P_FP64)
-            return launch_stage2d<EulerScheme<double>>(M3P_KIND_AMR_EULER2D, prm, TreeLoader<EulerScheme<double>>{u, recipes, n}, a, s);
-        return launch_stage2d<EulerScheme<float>>(M3P_KIND_AMR_EULER2D, prm, TreeLoader<EulerScheme<float>>{u, recipes, n}, a, s);
+        int st = precision == M3P_FP64
+               ? launch_stage2d<EulerScheme<double>>(M3P_KIND_AMR_EULER2D, prm, TreeLoader<EulerScheme<double>>{u, recipes, n}, a, s)
+               : launch_stage2d<EulerScheme<float>>(M3P_KIND_AMR_EULER2D, prm, TreeLoader<EulerScheme<float>>{u, recipes, n}, a, s);
+        if (st || !flux_reg) return st;
+        const long threads = (long)nleaf * 2 * n;
+        const double w = u0 ? 1.0 - rk_b : 1.0;
+        for (int phase = 0; phase < 2; ++phase)
+            hipLaunchKernelGGL(amr_reflux_kernel, dim3((threads + 255) / 256), dim3(256), 0, s, out, (const double*)flux_reg,
+                               recipes, dl_leaf, nleaf, n, dt, dt_dev, w, phase);
+        return launch_status("amr_reflux_kernel");
     }
     // two passes: extended primitive blocks into pe_work, then the update
     const long ext = (long)(n + 4) * (n + 4) * nleaf;
@@ -828,6 +835,16 @@ int m3p_amr_euler2d_stage(const double* u, const double* u0, double* out, double
 {
     return amr_euler2d_stage(u, u0, out, pe_work, recipes, dl_leaf, nleaf, n, dt, dt_dev, rk_b, plm_theta,
                              gamma_law_index, hllc, precision, stream, nullptr);
+}
+
+int m3p_amr_euler2d_stage_conservative(const double* u, const double* u0, double* out, double* flux_work,
+                                       const int* recipes, const double* dl_leaf, int nleaf, int n, double dt,
+                                       const double* dt_dev, double rk_b, double plm_theta, double gamma_law_index,
+                                       int hllc, int precision, void* stream)
+{
+    if (!flux_work) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_amr_euler2d_stage_conservative: NULL flux workspace");
+    return amr_euler2d_stage(u, u0, out, nullptr, recipes, dl_leaf, nleaf, n, dt, dt_dev, rk_b, plm_theta,
+                             gamma_law_index, hllc, precision, stream, nullptr, flux_work);
 }
 
 // Stages with the CFL reduction of their output fused in: the rate slot is

@@ -22,6 +22,14 @@ template<typename R> struct is_fp64 : std::is_same<R, double> {};
 // Generalised-minmod PLM slope, src/scheme_plm_gradient.hpp:59-67 with
 // coordinates (0, 1, 2).  The divisions by 1.0 and 2.0 there are exact, so
 // they are written as the identical-valued forms below.
+//
+// The reference's 0.25 |sa + sb| (sa + sc) min(|a|, |b|, |c|) (s = sign) is
+// sign(a) min|.| when a, b, c share a sign (the factors are 0.25 * 2 * 2,
+// exact) and zero otherwise: the value of the median of (min(a, b, c),
+// max(a, b, c), 0), which FP64 takes as max(min, min(max, 0)) -- the same
+// number in every case (a zero may differ only in its sign, which no sum
+// or product that follows can tell apart from the other zero's, since the
+// face state adds it to a nonzero primitive or the product is zero anyway).
 template<typename R>
 __device__ __forceinline__ R plm(R yl, R y0, R yr, R theta)
 {
@@ -29,9 +37,8 @@ __device__ __forceinline__ R plm(R yl, R y0, R yr, R theta)
     R b = (yr - yl) * R(0.5);
     R c = (yr - y0) * theta;
     if constexpr (is_fp64<R>::value) {
-        R sa = copysign(R(1), a), sb = copysign(R(1), b), sc = copysign(R(1), c);
-        R m = smin(smin(fabs(a), fabs(b)), fabs(c));
-        return R(0.25) * fabs(sa + sb) * (sa + sc) * m;
+        R mn = fmin(fmin(a, b), c), mx = fmax(fmax(a, b), c);
+        return fmax(mn, fmin(mx, R(0)));
     } else {
         // Same value: non-zero only when a, b, c share a sign; then sign(a) * min|.|.
         R m = fminf(fminf(fabsf(a), fabsf(b)), fabsf(c));

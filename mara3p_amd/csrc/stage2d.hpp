@@ -278,7 +278,19 @@ struct StageArgs {
     // fused CFL reduction (schemes with kCfl): atomic max of the rate
     // max(|v1|, |v2|) + c_s over dl of the values written, or nullptr
     unsigned long long* rate_slot;
+    // the face fluxes on the four edges of every block, [nblk][4][NC][n]
+    // (edges -x, +x, -y, +y; index along the edge), for the flux correction
+    // at quad-tree level jumps, or nullptr
+    double* flux_reg;
 };
+
+// Edge face flux of block b into the flux register (edge e, position i).
+template<int NC, typename R>
+__device__ __forceinline__ void store_edge_flux(double* reg, int b, int e, int n, int i, const R* f)
+{
+#pragma unroll
+    for (int m = 0; m < NC; ++m) reg[(((size_t)b * 4 + e) * NC + m) * n + i] = double(f[m]);
+}
 
 // KEEPU: keep each zone's conserved state in a three-row register window from
 // its load to its update (no second read of the state); otherwise re-load it
@@ -355,6 +367,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
         S::template face<2>(pl, pr, prm, td, tc, tc.k0 + r, tc.c0 + TJ, f);
 #pragma unroll
         for (int m = 0; m < NC; ++m) hF[m][r] = f[m];
+        if (a.flux_reg && tc.c0 + TJ == n) store_edge_flux<NC>(a.flux_reg, b, 3, n, tc.k0 + r, f);
     }
 
     // ---- march ------------------------------------------------------------
@@ -404,9 +417,11 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
                 }
             }
             R f[NC];
-            S::template face<1>(pl, pr, prm, td, tc, D > 0 ? kfirst : kfirst + 1, c, f);
+            const int fw = D > 0 ? kfirst : kfirst + 1;
+            S::template face<1>(pl, pr, prm, td, tc, fw, c, f);
 #pragma unroll
             for (int m = 0; m < NC; ++m) Fb[m] = double(f[m]);
+            if (a.flux_reg && (fw == 0 || fw == n)) store_edge_flux<NC>(a.flux_reg, b, fw == 0 ? 0 : 1, n, c, f);
         }
         __syncthreads();
 
@@ -459,7 +474,9 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
                     pr[m] = Pa[m] - ga[m];
                 }
             }
-            S::template face<1>(pl, pr, prm, td, tc, D > 0 ? k + 1 : k, c, Fa);
+            const int fk = D > 0 ? k + 1 : k;
+            S::template face<1>(pl, pr, prm, td, tc, fk, c, Fa);
+            if (a.flux_reg && (fk == 0 || fk == n)) store_edge_flux<NC>(a.flux_reg, b, fk == 0 ? 0 : 1, n, c, Fa);
 
             // The two edge lanes also store the out-of-tile neighbours of this
             // row (from the pre-pass) at the ends of the LDS rows, so every
@@ -489,6 +506,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             S::template face<2>(pl, pr, prm, td, tc, k, c, Fy);
 #pragma unroll
             for (int m = 0; m < NC; ++m) rowF[m][t] = Fy[m];
+            if (a.flux_reg && c == 0) store_edge_flux<NC>(a.flux_reg, b, 2, n, k, Fy);
             if (t == TJ - 1) {
 #pragma unroll
                 for (int m = 0; m < NC; ++m) rowF[m][TJ] = hF[m][kl];

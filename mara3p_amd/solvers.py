@@ -196,11 +196,16 @@ class AmrEuler2DSolver(_StateSolver):
     over amr::extend_block, src/mesh_amr.hpp:300-393).  All leaves advance with
     one dt; each uses its own level's cell spacing.  fused=True: one launch
     per RK stage reads guard zones through the leaf recipes; fused=False: a
-    guard-fill launch writes extended primitive blocks, then the update."""
+    guard-fill launch writes extended primitive blocks, then the update.
+    conservative=True (fused only): the flux through every edge between a
+    leaf and finer leaves is corrected to the finer leaves' face fluxes, so
+    the update conserves across level jumps (the reference does not)."""
     nc = 5
 
     def __init__(self, tree: QuadTree, block_size, gamma=5.0 / 3.0, plm_theta=1.5, precision="fp64",
-                 hllc=False, domain_size=1.0, fused=True, device="cuda"):
+                 hllc=False, domain_size=1.0, fused=True, conservative=False, device="cuda"):
+        if conservative and not fused:
+            raise ValueError("AmrEuler2DSolver: the conservative mode runs the fused stage")
         self.tree = tree
         self.n = int(block_size)
         self.nleaf = len(tree.leaves)
@@ -214,8 +219,18 @@ class AmrEuler2DSolver(_StateSolver):
         self.dl_host = domain_size / float(n) / (2.0 ** tree.levels().astype(np.float64))
         self.dl = torch.as_tensor(self.dl_host, dtype=torch.float64, device=self.device)
         self.dt_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self.flux_work = (torch.zeros((self.nleaf, 4, 5, n), dtype=torch.float64, device=self.device)
+                          if conservative else None)
 
     def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None, cfl=None, dt_next=None):
+        if self.flux_work is not None:
+            if cfl is not None:
+                raise ValueError("AmrEuler2DSolver: the conservative mode has no fused CFL stage")
+            check(lib().m3p_amr_euler2d_stage_conservative(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.flux_work),
+                                                           _ptr(self.recipes), _ptr(self.dl), self.nleaf, self.n, dt,
+                                                           _ptr(dt_dev), rk_b, self.theta, self.gamma, self.hllc,
+                                                           self.precision, _stream()))
+            return
         if cfl is not None:
             check(lib().m3p_amr_euler2d_stage_cfl(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.pe), _ptr(self.recipes),
                                                   _ptr(self.dl), self.nleaf, self.n, dt, _ptr(dt_dev), rk_b,

@@ -121,3 +121,83 @@ def rk2(u_ref, leaves, levels, n, dt, theta, gamma, hllc=0, steps=1):
         u2 = stage(u1, leaves, levels, n, dt, theta, gamma, hllc)
         u = u * 0.5 + u2 * (1.0 - 0.5)
     return u
+
+
+# --------------------------------------------------------------------------- conservative mode
+def edge_fluxes(pe_k, n, theta, gamma, hllc=0):
+    """Face fluxes on the four edges of one leaf from its extended primitive
+    block [n+4][n+4][5]: [4][n][5] for the edges -x (face row 0), +x (face
+    row n), -y (face column 0), +y (face column n), each face from the PLM
+    states lower + g/2 and upper - g/2 (src/module_euler.cpp:274-295)."""
+    riemann = orc().orc_euler_riemann_hllc if hllc else orc().orc_euler_riemann_hlle
+    plm = orc().orc_plm
+    out = np.zeros((4, n, 5))
+
+    def face(c0, c1, c2, c3, axis):
+        pl = np.array([c1[q] + 0.5 * plm(c0[q], c1[q], c2[q], theta) for q in range(5)])
+        pr = np.array([c2[q] - 0.5 * plm(c1[q], c2[q], c3[q], theta) for q in range(5)])
+        f = np.zeros(5)
+        riemann(P(pl), P(pr), axis, gamma, P(f))
+        return f
+
+    for i in range(n):
+        e = i + 2
+        out[0, i] = face(pe_k[0, e], pe_k[1, e], pe_k[2, e], pe_k[3, e], 1)
+        out[1, i] = face(pe_k[n, e], pe_k[n + 1, e], pe_k[n + 2, e], pe_k[n + 3, e], 1)
+        out[2, i] = face(pe_k[e, 0], pe_k[e, 1], pe_k[e, 2], pe_k[e, 3], 2)
+        out[3, i] = face(pe_k[e, n], pe_k[e, n + 1], pe_k[e, n + 2], pe_k[e, n + 3], 2)
+    return out
+
+
+def apply_flux_correction(out, fluxes, leaves, levels, n, dt, w, domain_size=1.0):
+    """For every leaf edge bordering finer leaves, replace the leaf's flux
+    through each edge face by the mean of the two finer face fluxes covering
+    it: the zone next to the edge changes by s dt / dl (mean - own) * w,
+    s = +1 on the lower edges, -1 on the upper; x edges first, then y edges.
+    out: [nleaf][n][n][5] (modified); fluxes: [nleaf][4][n][5]."""
+    index = {b: k for k, b in enumerate(leaves)}
+    for axis in (0, 1):
+        for k, b in enumerate(leaves):
+            dl = domain_size / float(n) / float(1 << int(levels[k]))
+            for side in (-1, 1):
+                nb = _neighbour(b, axis, side)
+                l, i, j = nb
+                if nb in index or (l > 0 and (l - 1, i >> 1, j >> 1) in index):
+                    continue
+                e = 2 * axis + (side > 0)
+                across = 1 if side < 0 else 0          # the children's position facing this leaf
+                for m in range(n):
+                    along = (2 * m) // n
+                    fi = 2 * m - along * n
+                    kid = (l + 1, 2 * i + across, 2 * j + along) if axis == 0 else \
+                          (l + 1, 2 * i + along, 2 * j + across)
+                    fe = 2 * axis + (side < 0)         # the child's edge facing this leaf
+                    fr = fluxes[index[kid], fe]
+                    mean = 0.5 * (fr[fi] + fr[fi + 1])
+                    own = fluxes[k, e, m]
+                    corr = (1.0 if side < 0 else -1.0) * ((mean - own) * dt / dl)
+                    r, c = (0 if side < 0 else n - 1, m) if axis == 0 else (m, 0 if side < 0 else n - 1)
+                    out[k, r, c] = out[k, r, c] + corr * w
+    return out
+
+
+def stage_conservative(u_ref, leaves, levels, n, dt, theta, gamma, hllc=0, domain_size=1.0):
+    """stage() followed by the flux correction at level jumps."""
+    out = stage(u_ref, leaves, levels, n, dt, theta, gamma, hllc, domain_size)
+    pe = extend_leaves(recover_primitives(u_ref, gamma), leaves, n)
+    fl = np.array([edge_fluxes(pe[k], n, theta, gamma, hllc) for k in range(len(leaves))])
+    return apply_flux_correction(out, fl, leaves, levels, n, dt, 1.0, domain_size)
+
+
+def rk2_conservative(u_ref, leaves, levels, n, dt, theta, gamma, hllc=0, steps=1):
+    """RK2 of the conservative stages; the second stage's correction is
+    applied after the combine u * 0.5 + L(u1) * (1 - 0.5), weighted 1 - 0.5."""
+    u = u_ref
+    for _ in range(steps):
+        u1 = stage_conservative(u, leaves, levels, n, dt, theta, gamma, hllc)
+        u2 = stage(u1, leaves, levels, n, dt, theta, gamma, hllc)
+        comb = u * 0.5 + u2 * (1.0 - 0.5)
+        pe = extend_leaves(recover_primitives(u1, gamma), leaves, n)
+        fl = np.array([edge_fluxes(pe[k], n, theta, gamma, hllc) for k in range(len(leaves))])
+        u = apply_flux_correction(comb, fl, leaves, levels, n, dt, 1.0 - 0.5)
+    return u

@@ -177,6 +177,66 @@ def test_oracle_stage_keeps_uniform_state_across_level_jumps():
 
 
 # --------------------------------------------------------------------------- GPU
+def _weighted_totals(u_ref, tree, n):
+    """Sum of each conserved component times the zone area, over all leaves
+    (math.fsum: exactly rounded)."""
+    import math
+    dl = 1.0 / n / (2.0 ** tree.levels().astype(float))
+    w = np.concatenate([np.full(n * n, d * d) for d in dl])
+    return np.array([math.fsum(u_ref[..., q].reshape(-1) * w) for q in range(5)])
+
+
+@pytest.mark.parametrize("hllc", [0, 1])
+def test_oracle_flux_correction_conserves(hllc):
+    """The oracle's conservative stage (flux correction at level jumps, the
+    optional mode the reference lacks): on a tree with jumps both ways, the
+    area-weighted totals of all five conserved quantities are unchanged to
+    round-off by one stage (the domain is periodic), while the reference's
+    uncorrected stage changes them; on a tree without jumps the two stages
+    are the same."""
+    n = 16
+    t = offset_tree(4)
+    assert len(set(t.levels())) >= 3
+    u = _kh_leaves(t, n)
+    tot0 = _weighted_totals(u, t, n)
+    scale = np.maximum(_weighted_totals(np.abs(u), t, n), 1e-300)
+    dt = 2e-3
+    cons = oamr.stage_conservative(u, t.leaves, t.levels(), n, dt, 1.5, G, hllc)
+    plain = oamr.stage(u, t.leaves, t.levels(), n, dt, 1.5, G, hllc)
+    d_cons = np.abs(_weighted_totals(cons, t, n) - tot0) / scale
+    d_plain = np.abs(_weighted_totals(plain, t, n) - tot0) / scale
+    assert d_cons.max() <= 1e-15, d_cons
+    assert d_plain[0] > 1e-9, d_plain
+    flat = QuadTree([(2, i, j) for i in range(4) for j in range(4)])
+    uf = _kh_leaves(flat, n)
+    assert np.array_equal(oamr.stage_conservative(uf, flat.leaves, flat.levels(), n, dt, 1.5, G, hllc),
+                          oamr.stage(uf, flat.leaves, flat.levels(), n, dt, 1.5, G, hllc))
+
+
+def test_oracle_flux_correction_touches_only_jump_edges():
+    """Only zones next to an edge that borders finer leaves change."""
+    n = 16
+    t = offset_tree(4)
+    u = _kh_leaves(t, n)
+    cons = oamr.stage_conservative(u, t.leaves, t.levels(), n, 1e-3, 1.5, G)
+    plain = oamr.stage(u, t.leaves, t.levels(), n, 1e-3, 1.5, G)
+    changed = np.any(cons != plain, axis=-1)
+    rec = t.recipes()
+    for k in range(len(t.leaves)):
+        allowed = np.zeros((n, n), bool)
+        if rec[k, 0, 0] == amr.FINE:
+            allowed[0, :] = True
+        if rec[k, 1, 0] == amr.FINE:
+            allowed[n - 1, :] = True
+        if rec[k, 2, 0] == amr.FINE:
+            allowed[:, 0] = True
+        if rec[k, 3, 0] == amr.FINE:
+            allowed[:, n - 1] = True
+        assert not np.any(changed[k] & ~allowed)
+        if allowed.any():
+            assert changed[k][allowed].any()
+
+
 torch = pytest.importorskip("torch")
 gpu = pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a HIP device")
 
@@ -303,3 +363,64 @@ def test_gpu_graph_replay_matches_eager():
         _ = (eager.u * 2.0).sum().item()        # unrelated eager work between replays
         assert torch.equal(graphed.u, eager.u)
         assert torch.equal(graphed.dt_dev, eager.dt_dev)
+
+
+@pytest.mark.gpu
+@gpu
+@pytest.mark.parametrize("hllc", [0, 1])
+def test_gpu_amr_conservative_fp64_matches_oracle(hllc):
+    """The conservative quad-tree stage (stage kernel writing the edge fluxes,
+    then the correction kernels) equals the oracle's conservative RK2 bit for
+    bit, two steps on a tree with jumps both ways."""
+    from mara3p_amd.solvers import AmrEuler2DSolver
+    n = 16
+    t = offset_tree(5)
+    u = _kh_leaves(t, n)
+    s = AmrEuler2DSolver(t, n, gamma=G, hllc=bool(hllc), conservative=True)
+    s.set_state(_to_dev_layout(u))
+    dt = 2e-4
+    s.step(dt)
+    s.step(dt)
+    s.check_errors()
+    ref = oamr.rk2_conservative(u, t.leaves, t.levels(), n, dt, 1.5, G, hllc, steps=2)
+    assert np.array_equal(_to_ref_layout(s.state()), ref)
+
+
+@pytest.mark.gpu
+@gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("state", ["sedov", "shear"])
+def test_gpu_amr_conservative_full_size(precision, state):
+    """The Sedov configuration's tree at its full size (1408 leaves of 64^2,
+    levels 3-6), 20 CFL-limited RK2 steps of the Sedov blast and of a shear
+    flow that crosses every level jump: in the conservative mode the
+    area-weighted mass, momenta and energy are conserved to round-off
+    (<= 1e-13 of the totals of |U|); with the shear flow the reference's
+    uncorrected mode drifts."""
+    from mara3p_amd.problems import sedov_amr
+    from mara3p_amd.solvers import AmrEuler2DSolver
+    n = 64
+    t = amr.nested_tree(6, 1.5)
+    assert len(t.leaves) == 1408
+    u0 = sedov_amr(t, n) if state == "sedov" else _to_dev_layout(_kh_leaves(t, n))
+    ref0 = _to_ref_layout(u0)
+    tot0 = _weighted_totals(ref0, t, n)
+    drift = {}
+    for conservative in (True, False):
+        s = AmrEuler2DSolver(t, n, gamma=G, precision=precision, conservative=conservative)
+        s.set_state(u0)
+        for _ in range(20):
+            dt = float(s.cfl_dt(0.4).item())
+            s.step(dt)
+        s.check_errors()
+        u = _to_ref_layout(s.state())
+        # relative to the totals of |U| (at the start or at the end: the
+        # Sedov momenta start at zero)
+        scale = np.maximum(np.maximum(_weighted_totals(np.abs(ref0), t, n), _weighted_totals(np.abs(u), t, n)),
+                           1e-300)
+        drift[conservative] = np.abs(_weighted_totals(u, t, n) - tot0) / scale
+    print("%s AMR %s relative drift of the totals: conservative %s, reference mode %s"
+          % (state, precision, drift[True], drift[False]))
+    assert drift[True].max() <= 1e-13, drift[True]
+    if state == "shear":
+        assert drift[False][0] > 1e-12, drift[False]
