@@ -10,7 +10,9 @@ two after --l1-steps steps is reported.
 
 One "step" = one RK2 step (two fused stage launches) over the whole mesh,
 state resident in HBM.  A zone-update is one zone through one RK2 step
-(the reference's kzps convention, src/app_problem.cpp:84-88).
+(the reference's kzps convention, src/app_problem.cpp:84-88).  Before the
+W warmup steps, untimed steps run for --settle-seconds (0.5 s) so that the
+K timed steps see the GPU at its working clocks.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload minidisk|kh|sedov_amr|ensemble|mhd]
 
@@ -44,6 +46,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", choices=["minidisk", "kh", "sedov_amr", "ensemble", "mhd"], default="minidisk")
     ap.add_argument("--l1-steps", type=int, default=1000)
+    ap.add_argument("--settle-seconds", type=float, default=0.5,
+                    help="untimed steps for this long before the warmup steps (GPU clocks settle)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--autotune", action="store_true",
@@ -167,11 +171,29 @@ def tile_in_use(workload, precision):
     return [tj.value, h.value]
 
 
-def timed_run(s, dt_of, steps, warmup, autotune=False):
+def settle(step, seconds, count=None):
+    """Untimed steps for `seconds` of wall time (or exactly `count` steps),
+    before the warmup steps: a timed region of a few milliseconds that starts
+    after an idle stretch (the solver's set-up on the host) measured its
+    kernels up to 10 % slower than the same kernels at working clocks.
+    Returns the number of steps taken."""
+    import torch
+    n, t0 = 0, time.perf_counter()
+    while (n < count) if count is not None else (time.perf_counter() - t0 < seconds):
+        step()
+        n += 1
+        if n % 8 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    return n
+
+
+def timed_run(s, dt_of, steps, warmup, autotune=False, settle_seconds=0.0):
     """Time `steps` RK2 steps; per-launch durations from HIP events on the launch stream."""
     import torch
     if autotune and hasattr(s, "autotune"):
         s.autotune()
+    settle(lambda: s.step(dt_of(getattr(s, "time", 1.0))), settle_seconds)
     for _ in range(warmup):
         s.step(dt_of(getattr(s, "time", 1.0)))
     torch.cuda.synchronize()
@@ -335,9 +357,13 @@ def bench_mhd(args):
     N = 256
     uc, b1, b2, b3 = problems.orszag_tang_3d(N, seed=1)
     out, final = {}, {}
+    settled = None
     for precision in ("mixed", "fp64"):
         s = MhdSolver(N, precision=precision)
         s.set_state(uc, b1, b2, b3)
+        # the FP64 solver takes as many settling steps as the mixed one, so
+        # that both end after the same number of steps for the L1 comparison
+        settled = settle(s.step, args.settle_seconds, settled)
         for _ in range(args.warmup):
             s.step()
         torch.cuda.synchronize()
@@ -364,7 +390,8 @@ def bench_mhd(args):
         torch.cuda.synchronize()
     m = out["mixed"]
     result = {"metric": "zone-updates/s (3D MHD CT)", "value": m["value"], "unit": "zone-updates/s", "n_gpus": 1,
-              "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True,
+              "steps": args.steps, "warmup": args.warmup, "settle_s": args.settle_seconds,
+              "ms_per_step": m["ms_per_step"], "higher_is_better": True,
               "scaling": "weak", "vs_baseline": None, "dtype": "f32-flux/f64-state", "data": "synthetic",
               "config": {"workload": "ideal MHD CT, Orszag-Tang 256^3 periodic, PLM+HLLE, RK2", "zones": N ** 3},
               "roofline": {"bound": "hbm", "achieved": m["hbm_frac"] * HBM_PEAK_GBS, "peak": HBM_PEAK_GBS,
@@ -372,7 +399,7 @@ def bench_mhd(args):
                            "note": "per RK2 step over all kernels of both stages (achieved and traffic)"},
               "fp64": out["fp64"], "max_div_b_mixed": m["max_div_b"]}
     if final:
-        # both precisions took warmup + steps RK2 steps, each with its own CFL time step per stage
+        # both precisions took settle + warmup + steps RK2 steps, each with its own CFL time step per stage
         names = ["D", "S1", "S2", "S3", "E", "B1", "B2", "B3"]
         fa, fb = final["fp64"], final["mixed"]
         arrays = [(fa[0][q], fb[0][q]) for q in range(5)] + [(fa[1 + a], fb[1 + a]) for a in range(3)]
@@ -380,7 +407,7 @@ def bench_mhd(args):
         for name, (x, y) in zip(names, arrays):
             d = float(np.mean(np.abs(x - y)))
             per[name] = {"l1": d, "l1_rel": d / max(float(np.mean(np.abs(x))), 1e-300)}
-        result["l1_mixed_vs_fp64"] = {"steps": args.warmup + args.steps, "per_variable": per}
+        result["l1_mixed_vs_fp64"] = {"steps": settled + args.warmup + args.steps, "per_variable": per}
     print(json.dumps(result), flush=True)
 
 
@@ -409,7 +436,7 @@ def main():
         rep = l1_report(args.workload, steps)
     s, u0, dt_of, nc = make_solver(args.workload, "mixed")
     zones = s.zones
-    wall, d1, d2 = timed_run(s, dt_of, args.steps, args.warmup, args.autotune)
+    wall, d1, d2 = timed_run(s, dt_of, args.steps, args.warmup, args.autotune, args.settle_seconds)
     tile = tile_in_use(args.workload, "mixed")
     graph = (graph_run(s, args.steps, args.warmup)
              if args.workload in ("kh", "sedov_amr", "ensemble") and not args.no_graph else None)
@@ -426,6 +453,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "settle_s": args.settle_seconds,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -459,7 +487,7 @@ def main():
     if not args.no_fp64:
         s64, _, _, _ = make_solver(args.workload, "fp64")
         k64 = max(10, args.steps // 4)
-        wall64, e1, e2 = timed_run(s64, dt_of, k64, max(2, args.warmup // 4), args.autotune)
+        wall64, e1, e2 = timed_run(s64, dt_of, k64, max(2, args.warmup // 4), args.autotune, args.settle_seconds)
         tile64 = tile_in_use(args.workload, "fp64")
         v64 = zones * k64 / wall64
         result["fp64"] = {"value": v64, "ms_per_step": 1e3 * wall64 / k64, "stage_ms": {"stage1": e1, "stage2": e2},

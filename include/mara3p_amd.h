@@ -193,7 +193,8 @@ int  m3p_mhd_advance_ct(const double* uc, const double* bf1, const double* bf2, 
  * m3p_mhd_advance_ct, with no workspace argument: the workspace is a static
  * device array of 48 MiB inside the library (one per GPU, held until the
  * process ends), so N is limited to what fits (m3p_mhd_workspace_bytes(N,
- * precision) <= 48 MiB: N <= 59 in FP64, <= 75 mixed); a larger N fails with
+ * precision) <= 48 MiB: N a multiple of 8, whose passes keep no face fluxes,
+ * up to 72 in FP64 and 96 mixed; any other N up to 59 and 75); a larger N fails with
  * M3P_ERR_BAD_ARGUMENT.  Kernel errors (negative density or pressure) go to
  * status, a caller's device int[4] (code, block, zone) that the call clears
  * first; m3p_check_errors does not see them.  Nothing is allocated, with

@@ -4,11 +4,17 @@
 
 namespace m3p {
 
+// Cubes with N % 8 == 0 take the column-tile pass, whose face fluxes never
+// leave the workgroup; other N write them, [5][nf] per face family.
+static bool mhd_tiled(int N) { return N % 8 == 0 && N >= 8; }
+
 long mhd_workspace_bytes(int N, int precision)
 {
     const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
     const long rs = precision == M3P_FP64 ? 8 : 4;
-    return (8 * N3 + 3 * 7 * nf) * rs + 1024;    // + alignment, the speed slots and dt
+    // primitives [8][N^3], face fields [2][nf] per family, face fluxes (per-cell
+    // passes only), + alignment, the speed slots and dt
+    return (8 * N3 + 3 * 2 * nf + (mhd_tiled(N) ? 0 : 3 * 5 * nf)) * rs + 1024;
 }
 
 template<typename R>
@@ -16,14 +22,15 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
 {
     const int N = a.N;
     const long N3 = (long)N * N * N, nf = (long)(N + 1) * N * N;
+    const bool tiled = mhd_tiled(N);
     R* P = reinterpret_cast<R*>(work);
-    R* F1 = P + 8 * N3;
-    R* F2 = F1 + 5 * nf;
-    R* F3 = F2 + 5 * nf;
-    R* E1 = F3 + 5 * nf;
+    R* E1 = P + 8 * N3;
     R* E2 = E1 + 2 * nf;
     R* E3 = E2 + 2 * nf;
-    char* tail = reinterpret_cast<char*>(E3 + 2 * nf);
+    R* F1 = E3 + 2 * nf;                     // per-cell passes only
+    R* F2 = F1 + 5 * nf;
+    R* F3 = F2 + 5 * nf;
+    char* tail = reinterpret_cast<char*>(tiled ? F1 : F3 + 5 * nf);
     tail += (256 - (reinterpret_cast<uintptr_t>(tail) & 255)) & 255;
     a.vmax = reinterpret_cast<unsigned long long*>(tail);
     if (!a.dt) a.dt = reinterpret_cast<double*>(tail + 8 * kRateSlots);
@@ -32,10 +39,11 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
     hipLaunchKernelGGL(mhd_prim_kernel<R>, dim3(std::min(gc, 4096)), dim3(256), 0, s, a, P);
     hipLaunchKernelGGL(mhd_dt_kernel, dim3(1), dim3(kRateSlots), 0, s, a);
     FaceE<R> fe{ E1, E2, E3, N, nf };
-    if constexpr (!is_fp64<R>::value) if (N % 8 == 0 && N >= 8) {
+    if (tiled) {
         // column tiles: fluxes and the conserved update in one pass, then the
-        // faces (the FP64 tile needs ~255 registers and runs slower than the
-        // separate passes, so FP64 keeps those)
+        // faces.  FP64 too (244 VGPRs at two waves per SIMD, no spills): at
+        // 256^3 its RK2 step takes 5.77 ms against 7.06 ms through the
+        // per-cell passes below, which write every face flux and read it back.
         // 16-plane segments: 4096 workgroups at 256^3, four full rounds of
         // the 1024 resident at four waves per SIMD (32-plane segments run
         // as fast; 16 x 16 column tiles 10 % slower).  TI must divide N: an

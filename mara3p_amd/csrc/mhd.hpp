@@ -370,11 +370,15 @@ mhd_boundary_faces_kernel(MhdArgs a, const R* P, R* F1, R* E1, R* F2, R* E2, R* 
 // E arrays for the face update.  Every face is computed from the same
 // operands in the same order as mhd_one_face, so a face shared by two
 // tiles or segments has one value.
-// Four waves per SIMD: the register allocator then fits the kernel in 128
-// VGPRs without spills (150 when unconstrained, three waves), and the extra
-// wave hides load latency: 11 % faster per RK2 step at 256^3.
+// Mixed: four waves per SIMD, where the register allocator fits the kernel
+// in 128 VGPRs without spills (150 when unconstrained, three waves) and the
+// extra wave hides load latency (11 % faster per RK2 step at 256^3).  FP64:
+// two waves per SIMD (up to 256 VGPRs), the update in the arithmetic and
+// order of mhd_update_kernel.
+template<typename R> struct MhdTileWaves { static constexpr int value = is_fp64<R>::value ? 2 : 4; };
+
 template<typename R, int TJ, int TK>
-__global__ void __launch_bounds__(TJ * TK) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(TJ * TK) __attribute__((amdgpu_waves_per_eu(MhdTileWaves<R>::value, 8)))
 mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
 {
     constexpr int NT = TJ * TK;
@@ -521,8 +525,14 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
                 double d1 = double(Fx_hi[q]) - double(Fx_lo[q]);
                 double d2 = double(F2s[q][jl + 1][kl]) - double(F2s[q][jl][kl]);
                 double d3 = double(F3s[q][jl][kl + 1]) - double(F3s[q][jl][kl]);
-                double r = fma(-(d1 + d2 + d3), dtdl, a.uc[q * N3 + c]);
-                if (a.u0) r = fma(a.u0[q * N3 + c], a.rk_b, r * (1.0 - a.rk_b));
+                double r;
+                if constexpr (is_fp64<R>::value) {
+                    r = a.uc[q * N3 + c] - (d1 + d2 + d3) * dt / a.dl;
+                    if (a.u0) r = a.u0[q * N3 + c] * a.rk_b + r * (1.0 - a.rk_b);
+                } else {
+                    r = fma(-(d1 + d2 + d3), dtdl, a.uc[q * N3 + c]);
+                    if (a.u0) r = fma(a.u0[q * N3 + c], a.rk_b, r * (1.0 - a.rk_b));
+                }
                 a.uo[q * N3 + c] = r;
             }
         }

@@ -106,6 +106,7 @@ struct MeshLoader {
     // row zone one iteration ahead, recover it when it is needed.
     static constexpr bool kPrefetch = true;
     static constexpr bool kUnroll = true;    // march unrolled by the scheme's kMarchUnroll
+    static constexpr bool kEdgeFluxes = false;   // no edge-flux register (uniform meshes)
     __device__ __forceinline__ void fetch_row(int b, int r, int c, double* raw) const { cons_row(b, r, c, raw); }
     __device__ __forceinline__ bool finish_row(int, int, int, const typename S::Params& prm, const double* raw,
                                                double* uu, typename S::Real* p) const
@@ -125,6 +126,7 @@ struct ExtendedLoader {
     int n;
     static constexpr bool kPrefetch = false;
     static constexpr bool kUnroll = false;
+    static constexpr bool kEdgeFluxes = false;
     __device__ __forceinline__ void bind(int) {}
 
     // Block b of a list of blocks: uc + b NC n^2, pe + b NC (n+4)^2.
@@ -244,6 +246,7 @@ struct TreeLoader {
     // assembled when it is needed.
     static constexpr bool kPrefetch = true;
     static constexpr bool kUnroll = false;
+    static constexpr bool kEdgeFluxes = true;    // a.flux_reg may be set (conservative quad-tree mode)
     __device__ __forceinline__ void fetch_row(int b, int r, int c, double* raw) const
     {
         if (r >= 0 && r < n) cons_row(b, r, c, raw);
@@ -367,7 +370,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
         S::template face<2>(pl, pr, prm, td, tc, tc.k0 + r, tc.c0 + TJ, f);
 #pragma unroll
         for (int m = 0; m < NC; ++m) hF[m][r] = f[m];
-        if (a.flux_reg && tc.c0 + TJ == n) store_edge_flux<NC>(a.flux_reg, b, 3, n, tc.k0 + r, f);
+        if (Ld::kEdgeFluxes && a.flux_reg && tc.c0 + TJ == n) store_edge_flux<NC>(a.flux_reg, b, 3, n, tc.k0 + r, f);
     }
 
     // ---- march ------------------------------------------------------------
@@ -421,7 +424,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             S::template face<1>(pl, pr, prm, td, tc, fw, c, f);
 #pragma unroll
             for (int m = 0; m < NC; ++m) Fb[m] = double(f[m]);
-            if (a.flux_reg && (fw == 0 || fw == n)) store_edge_flux<NC>(a.flux_reg, b, fw == 0 ? 0 : 1, n, c, f);
+            if (Ld::kEdgeFluxes && a.flux_reg && (fw == 0 || fw == n)) store_edge_flux<NC>(a.flux_reg, b, fw == 0 ? 0 : 1, n, c, f);
         }
         __syncthreads();
 
@@ -476,7 +479,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             }
             const int fk = D > 0 ? k + 1 : k;
             S::template face<1>(pl, pr, prm, td, tc, fk, c, Fa);
-            if (a.flux_reg && (fk == 0 || fk == n)) store_edge_flux<NC>(a.flux_reg, b, fk == 0 ? 0 : 1, n, c, Fa);
+            if (Ld::kEdgeFluxes && a.flux_reg && (fk == 0 || fk == n)) store_edge_flux<NC>(a.flux_reg, b, fk == 0 ? 0 : 1, n, c, Fa);
 
             // The two edge lanes also store the out-of-tile neighbours of this
             // row (from the pre-pass) at the ends of the LDS rows, so every
@@ -506,7 +509,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             S::template face<2>(pl, pr, prm, td, tc, k, c, Fy);
 #pragma unroll
             for (int m = 0; m < NC; ++m) rowF[m][t] = Fy[m];
-            if (a.flux_reg && c == 0) store_edge_flux<NC>(a.flux_reg, b, 2, n, k, Fy);
+            if (Ld::kEdgeFluxes && a.flux_reg && c == 0) store_edge_flux<NC>(a.flux_reg, b, 2, n, k, Fy);
             if (t == TJ - 1) {
 #pragma unroll
                 for (int m = 0; m < NC; ++m) rowF[m][TJ] = hF[m][kl];

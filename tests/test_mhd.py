@@ -111,8 +111,11 @@ def _rel(a, b):
 
 @pytest.mark.gpu
 @gpu
-@pytest.mark.parametrize("N", [8, 16, 128])
+@pytest.mark.parametrize("N", [8, 12, 16, 72, 128])
 def test_gpu_mhd_advance_ct_fp64_matches_oracle(N):
+    """Two FP64 RK2 steps against the oracle: N = 12 runs the per-cell passes,
+    8 and 16 the 8 x 8 column tiles, 128 the 8 x 32 tiles in 16-plane
+    segments, 72 (N % 16 == 8) the 8 x 8 tiles in 8-plane segments."""
     from mara3p_amd.problems import orszag_tang_3d
     from mara3p_amd.solvers import MhdSolver
     uc, b1, b2, b3 = orszag_tang_3d(N, seed=2, perturbation=0.05)

@@ -1,7 +1,7 @@
 """m3p_mhd_advance_ct_noalloc (the entry point that allocates nothing, its
 MHD workspace a static device array) against m3p_mhd_advance_ct with a
 caller workspace: the same kernels, so every output and the time step must
-be equal bit for bit, on two calls in a row, the second on a smaller cube
+be equal bit for bit, on calls in a row, each on a smaller cube
 (the static array keeps the first call's contents)."""
 import ctypes
 
@@ -48,7 +48,9 @@ def _advance(lib, src, base, N, precision, noalloc):
 def test_noalloc_mhd_equals_workspace_entry(precision):
     from mara3p_amd._lib import lib
     L = lib()
-    for N, seed in ((40, 1), (24, 2)):      # the second call on a smaller cube
+    # later calls on smaller cubes; 20 takes the per-cell passes, 40 and 24
+    # the column tiles
+    for N, seed in ((40, 1), (24, 2), (20, 3)):
         src = _state(N, seed)
         base = _state(N, seed + 10)
         for b in (None, base):               # first stage, then an RK combine
