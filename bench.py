@@ -54,7 +54,7 @@ def parse():
                     help="time every tile shape first and keep the fastest (default: the library's tile table)")
     ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-graph", action="store_true",
-                    help="skip the hipGraph replays of CFL-limited steps (kh, sedov_amr, ensemble)")
+                    help="skip the hipGraph replays (kh, sedov_amr, ensemble: CFL-limited steps; minidisk: device-clocked steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (for HBM counter calibration under rocprofv3)")
     return ap.parse_args()
@@ -256,6 +256,44 @@ def graph_run(s, steps, warmup, cfl=0.4):
                     "fused into each step's second stage, separate_cfl_ms_per_step with a reduction pass per step"}
 
 
+def graph_run_minidisk(s, steps, warmup, settle_seconds):
+    """The minidisk steps on the device clock (m3p_minidisk_clock: dt from the
+    reference driver's rule and the binary's state computed on the device)
+    captured in a hipGraph: one replay is two RK2 steps (a clock kernel and
+    two stage launches each), or two forward-Euler steps, the reference
+    driver's substep (a clock kernel and one stage launch each).  Whole-step
+    wall time over the replays."""
+    import torch
+    from mara3p_amd.solvers import Graph
+    reps = max(1, steps // 2)
+    out = {}
+    for order in (2, 1):
+        torch.cuda.synchronize()
+        stream = torch.cuda.Stream()       # capture needs a stream other than the default one
+        with torch.cuda.stream(stream):
+            s.set_clock(getattr(s, "time", 0.0))
+            with Graph() as g:
+                s.device_step_pair(order)
+            settle(g.replay, settle_seconds)
+            for _ in range(max(1, warmup // 2)):
+                g.replay()
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                g.replay()
+            stream.synchronize()
+            wall = time.perf_counter() - t0
+            s.check_errors()
+        torch.cuda.synchronize()
+        del g
+        out[order] = wall
+    return {"value": s.zones * 2 * reps / out[2], "ms_per_step": 1e3 * out[2] / (2 * reps), "steps": 2 * reps,
+            "forward_euler_ms_per_step": 1e3 * out[1] / (2 * reps),
+            "note": "hipGraph replays of two device-clocked RK2 steps (value, ms_per_step) or two forward-Euler "
+                    "steps (the reference driver's substep, one stage launch each); the clock kernel computes "
+                    "dt and the binary's state on the device"}
+
+
 def host_cpu():
     """The host CPU model and logical CPU count (from /proc/cpuinfo)."""
     model = "unknown"
@@ -438,8 +476,12 @@ def main():
     zones = s.zones
     wall, d1, d2 = timed_run(s, dt_of, args.steps, args.warmup, args.autotune, args.settle_seconds)
     tile = tile_in_use(args.workload, "mixed")
-    graph = (graph_run(s, args.steps, args.warmup)
-             if args.workload in ("kh", "sedov_amr", "ensemble") and not args.no_graph else None)
+    graph = None
+    if not args.no_graph:
+        if args.workload in ("kh", "sedov_amr", "ensemble"):
+            graph = graph_run(s, args.steps, args.warmup)
+        elif args.workload == "minidisk":
+            graph = graph_run_minidisk(s, args.steps, args.warmup, args.settle_seconds)
     ms_per_step = 1e3 * wall / args.steps
     value = zones * args.steps / wall
     # algorithmic bytes: stage 1 reads U0 writes U1 (2 nc 8 B/zone), stage 2 reads U1, U0 writes U (3 nc 8 B/zone)

@@ -162,6 +162,29 @@ int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int
                        int nblk, double time, double dt, double rk_b,
                        const m3p_minidisk_data* data /* host */, int precision, void* stream);
 
+/* A minidisk step with its clock in device memory, so that whole steps can be
+ * captured in a hipGraph and replayed (the stage above takes the time from
+ * the host).  m3p_minidisk_clock: the time step rule of the reference's
+ * minidisk driver (problems/minidisk.cpp:202-206: smallest_cell_size / vmax
+ * * cfl * min(1, 0.01 + t / 0.05)) and the binary's state
+ * (src/physics_two_body.hpp:109-157,417-421) evaluated by one device thread.
+ * clock [2] (device doubles) = {t, dt}: reads t, writes dt into clock[1],
+ * the binary's state {M1, x1, y1, M2, x2, y2} at t into orbit[0..5] and at
+ * t + dt into orbit[6..11] (device doubles), and the time after the step
+ * into clock[0]: t * 0.5 + (t + dt) * 0.5 for an RK2 step (rk_order 2, the
+ * solution-time combination of src/module_euler.cpp:357), t + dt for the
+ * reference driver's forward-Euler substep (rk_order 1,
+ * problems/minidisk.cpp:172-186).  The device's sin and cos may differ from
+ * the host's in the last bit, so the orbit may differ from the host stage's
+ * by an ulp.  m3p_minidisk_stage_dev: m3p_minidisk_stage with the binary's
+ * state read from orbit (6 device doubles: orbit for the first stage of a
+ * step, orbit + 6 for the second) and dt from dt_dev (clock + 1). */
+int m3p_minidisk_clock(const m3p_minidisk_data* data /* host */, double* clock, double* orbit, int rk_order,
+                       void* stream);
+int m3p_minidisk_stage_dev(const double* u, const double* u0, double* out, const int* nbr, const int* bcoord,
+                           int nblk, const double* orbit, const double* dt_dev, double rk_b,
+                           const m3p_minidisk_data* data /* host */, int precision, void* stream);
+
 /* 1D Euler stage over nblk blocks of n zones, out-flow at the two ends of the
  * chain described by nbr [nblk][2] = {prev, next} (-1 = outflow boundary). */
 int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int* nbr,

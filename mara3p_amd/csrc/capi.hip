@@ -291,6 +291,7 @@ MinidiskParams minidisk_params(const m3p_minidisk_data* d, double time)
     two_body_state(1.0, 1.0, d->mass_ratio, d->eccentricity, time, d->omega_frame, st);
     p.m1 = st[0]; p.x1 = st[1]; p.y1 = st[2];
     p.m2 = st[5]; p.x2 = st[6]; p.y2 = st[7];
+    p.orbit = nullptr;
     p.level = d->depth;
     return p;
 }
@@ -400,6 +401,29 @@ int scratch_address(void** out)
     }
     *out = g_scratch_ptr[dev];
     return M3P_OK;
+}
+
+// One minidisk step's clock (one thread): dt from the reference's rule at
+// t = clock[0] (problems/minidisk.cpp:202-206: smallest_cell_size / vmax *
+// cfl * min(1, 0.01 + t / 0.05), evaluated left to right), the binary's
+// state at t and at t + dt (the second stage's time) into orbit[0..5] and
+// orbit[6..11], and the time after the step into clock[0] (the RK2
+// combination t * 0.5 + (t + dt) * 0.5 of the solution time, or t + dt).
+__global__ void minidisk_clock_kernel(double dx_over_vmax, double cfl, double q, double e, double omega,
+                                      int rk_order, double* clock, double* orbit)
+{
+    if (threadIdx.x != 0) return;
+    const double t = clock[0];
+    const double dt = dx_over_vmax * (cfl * fmin(1.0, 0.01 + t / 0.05));
+    double st[10];
+    for (int k = 0; k < 2; ++k) {
+        two_body_state(1.0, 1.0, q, e, k == 0 ? t : t + dt, omega, st);
+        double* o = orbit + 6 * k;
+        o[0] = st[0]; o[1] = st[1]; o[2] = st[2];
+        o[3] = st[5]; o[4] = st[6]; o[5] = st[7];
+    }
+    clock[1] = dt;
+    clock[0] = rk_order == 2 ? t * 0.5 + (t + dt) * (1.0 - 0.5) : t + dt;
 }
 
 __global__ void clear_status_kernel(int* status)
@@ -529,9 +553,9 @@ int m3p_euler2d_stage(const double* u, const double* u0, double* out, const int*
                          precision, stream, nullptr);
 }
 
-int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int* nbr, const int* bcoord,
-                       int nblk, double time, double dt, double rk_b,
-                       const m3p_minidisk_data* data, int precision, void* stream)
+static int minidisk_stage(const double* u, const double* u0, double* out, const int* nbr, const int* bcoord,
+                          int nblk, double time, double dt, const double* orbit, const double* dt_dev, double rk_b,
+                          const m3p_minidisk_data* data, int precision, void* stream)
 {
     StreamState* ds;
     if (int st = stream_state(stream, &ds)) return st;
@@ -540,8 +564,9 @@ int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int
     if (u == out || (u0 && u0 == out)) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: stage output must not alias its inputs");
     const int n = data->block_size;
     MinidiskParams prm = minidisk_params(data, time);
+    prm.orbit = orbit;
     StageArgs a{};
-    a.u0 = u0; a.out = out; a.dt = dt; a.rk_b = rk_b; a.bcoord = bcoord;
+    a.u0 = u0; a.out = out; a.dt = dt; a.dt_dev = dt_dev; a.rk_b = rk_b; a.bcoord = bcoord;
     a.n = n; a.nblk = nblk; a.err = ds->err;
     a.dl = 2.0 * data->domain_radius / double(data->block_size) / double(1 << data->depth);
     hipStream_t s = (hipStream_t)stream;
@@ -553,6 +578,38 @@ int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int
     if (precision == M3P_FP64)
         return launch_stage2d<MinidiskScheme<double>>(M3P_KIND_MINIDISK, prm, MeshLoader<MinidiskScheme<double>>{u, nbr, n}, a, s);
     return launch_stage2d<MinidiskScheme<float>>(M3P_KIND_MINIDISK, prm, MeshLoader<MinidiskScheme<float>>{u, nbr, n}, a, s);
+}
+
+int m3p_minidisk_stage(const double* u, const double* u0, double* out, const int* nbr, const int* bcoord,
+                       int nblk, double time, double dt, double rk_b,
+                       const m3p_minidisk_data* data, int precision, void* stream)
+{
+    return minidisk_stage(u, u0, out, nbr, bcoord, nblk, time, dt, nullptr, nullptr, rk_b, data, precision, stream);
+}
+
+int m3p_minidisk_stage_dev(const double* u, const double* u0, double* out, const int* nbr, const int* bcoord,
+                           int nblk, const double* orbit, const double* dt_dev, double rk_b,
+                           const m3p_minidisk_data* data, int precision, void* stream)
+{
+    if (!orbit || !dt_dev) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_minidisk_stage_dev: orbit and dt_dev are required");
+    return minidisk_stage(u, u0, out, nbr, bcoord, nblk, 0.0, 0.0, orbit, dt_dev, rk_b, data, precision, stream);
+}
+
+int m3p_minidisk_clock(const m3p_minidisk_data* data, double* clock, double* orbit, int rk_order, void* stream)
+{
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
+    if (int st = check_minidisk_data(data)) return st;
+    if (!clock || !orbit || (rk_order != 1 && rk_order != 2))
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_minidisk_clock: bad arguments");
+    // smallest_cell_size / vmax on the host, in the reference's order
+    // (problems/minidisk_scheme.cpp:122-125, problems/minidisk.cpp:204-206)
+    const double dx = 2.0 * data->domain_radius / double(data->block_size) / double(1 << data->depth);
+    const double vmax = std::sqrt(1.0 * 0.5 / data->softening_length);
+    hipLaunchKernelGGL(minidisk_clock_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                       dx / vmax, data->cfl, data->mass_ratio, data->eccentricity, data->omega_frame, rk_order,
+                       clock, orbit);
+    return launch_status("m3p_minidisk_clock");
 }
 
 int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int* nbr,

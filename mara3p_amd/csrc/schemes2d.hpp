@@ -88,8 +88,21 @@ struct MinidiskParams {
     double inv_buffer_scale;
     double m1, x1, y1;   // binary components at this stage's time
     double m2, x2, y2;
+    // or, when set, the same six values {m1, x1, y1, m2, x2, y2} in device
+    // memory, written by the clock kernel of a device-clocked step
+    const double* orbit;
     int level;
 };
+
+// Stage kernels call this on their parameters first: a device-clocked
+// minidisk stage takes the binary's state from device memory.
+__device__ __forceinline__ void resolve_params(MinidiskParams& p)
+{
+    if (p.orbit) {
+        p.m1 = p.orbit[0]; p.x1 = p.orbit[1]; p.y1 = p.orbit[2];
+        p.m2 = p.orbit[3]; p.x2 = p.orbit[4]; p.y2 = p.orbit[5];
+    }
+}
 
 template<typename R>
 struct MinidiskScheme {

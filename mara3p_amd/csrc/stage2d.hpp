@@ -287,6 +287,11 @@ struct StageArgs {
     double* flux_reg;
 };
 
+// Parameters that a stage kernel takes as they are; a scheme whose
+// parameters may point at device memory overloads this (found by
+// argument-dependent lookup at the kernel's instantiation).
+template<class P> __device__ __forceinline__ void resolve_params(P&) {}
+
 // Edge face flux of block b into the flux register (edge e, position i).
 template<int NC, typename R>
 __device__ __forceinline__ void store_edge_flux(double* reg, int b, int e, int n, int i, const R* f)
@@ -306,6 +311,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
 {
     constexpr int NC = S::NC;
     using R = typename S::Real;
+    resolve_params(prm);
 
     const int n = a.n;
     const int tiles_i = n / H, tiles_j = n / TJ;

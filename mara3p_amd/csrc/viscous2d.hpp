@@ -63,6 +63,7 @@ iso_visc_stage_kernel(typename S::Params prm, Ld ld_arg, StageArgs a, double nu)
 {
     using R = typename S::Real;
     constexpr int NC = 3;
+    resolve_params(prm);
     const int n = a.n;
     const long zpb = (long)n * n;
     const long z = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -145,11 +145,52 @@ class MinidiskSolver(_StateSolver):
         self.nbr = torch.as_tensor(self.mesh.neighbours(), device=self.device)
         self.bcoord = torch.as_tensor(self.mesh.block_coordinates(), device=self.device)
         self.time = 0.0
+        # device-clocked steps: {t, dt} and the binary's state at the two stage times
+        self.clock = torch.zeros(2, dtype=torch.float64, device=self.device)
+        self.orbit = torch.zeros(12, dtype=torch.float64, device=self.device)
 
     def stage(self, u, u0, out, time, dt, rk_b=0.5):
         check(lib().m3p_minidisk_stage(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), _ptr(self.bcoord),
                                        self.mesh.nblk, time, dt, rk_b, ctypes.byref(self.data),
                                        self.precision, _stream()))
+
+    def stage_dev(self, u, u0, out, k, rk_b=0.5):
+        """A stage whose dt and binary state come from device memory (the
+        clock's dt, the orbit of stage k of the step)."""
+        orbit = ctypes.c_void_p(self.orbit.data_ptr() + 6 * 8 * k)
+        dt = ctypes.c_void_p(self.clock.data_ptr() + 8)
+        check(lib().m3p_minidisk_stage_dev(_ptr(u), _ptr(u0), _ptr(out), _ptr(self.nbr), _ptr(self.bcoord),
+                                           self.mesh.nblk, orbit, dt, rk_b, ctypes.byref(self.data),
+                                           self.precision, _stream()))
+
+    def set_clock(self, time):
+        """Start the device clock at `time` (the time step follows the
+        reference driver's rule, problems/minidisk.cpp:202-206)."""
+        self.clock.fill_(0.0)
+        self.clock[0] = time
+
+    def clock_time(self):
+        """The device clock's time and last time step."""
+        t, dt = self.clock.cpu().tolist()
+        return t, dt
+
+    def device_step(self, u, out, rk_order=2):
+        """One step from u into out on the device clock: the clock kernel,
+        then two stages (RK2) or one (forward Euler, the reference driver's
+        substep, problems/minidisk.cpp:172-186)."""
+        check(lib().m3p_minidisk_clock(ctypes.byref(self.data), _ptr(self.clock), _ptr(self.orbit), rk_order,
+                                       _stream()))
+        if rk_order == 2:
+            self.stage_dev(u, None, self.u1, 0)
+            self.stage_dev(self.u1, u, out, 1)
+        else:
+            self.stage_dev(u, None, out, 0)
+
+    def device_step_pair(self, rk_order=2):
+        """Two device-clocked steps that end in the buffer they began in
+        (u -> u2 -> u): the unit a Graph captures."""
+        self.device_step(self.u, self.u2, rk_order)
+        self.device_step(self.u2, self.u, rk_order)
 
     def autotune(self, reps=5):
         """Pick the fastest compiled tile shape for this mesh (blocking); returns (tj, h, ms)."""
