@@ -256,6 +256,38 @@ def graph_run(s, steps, warmup, cfl=0.4):
                     "fused into each step's second stage, separate_cfl_ms_per_step with a reduction pass per step"}
 
 
+def fused_run(s, dt_of, steps, warmup, settle_seconds):
+    """K RK2 steps through the fused kernel (one launch per step, the first
+    stage's state never stored): wall time and the mean launch duration from
+    HIP events on the launch stream."""
+    import torch
+    settle(lambda: s.step_fused(dt_of(s.time)), settle_seconds)
+    for _ in range(warmup):
+        s.step_fused(dt_of(s.time))
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    orig = s.rk2_step
+    counter = [0]
+
+    def rk2_step(*a, **k):
+        e0, e1 = ev[counter[0]]
+        e0.record(stream)
+        orig(*a, **k)
+        e1.record(stream)
+        counter[0] += 1
+    s.rk2_step = rk2_step
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step_fused(dt_of(s.time))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s.rk2_step = orig
+    s.check_errors()
+    return wall, float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+
 def graph_run_minidisk(s, steps, warmup, settle_seconds):
     """The minidisk steps on the device clock (m3p_minidisk_clock: dt from the
     reference driver's rule and the binary's state computed on the device)
@@ -267,13 +299,13 @@ def graph_run_minidisk(s, steps, warmup, settle_seconds):
     from mara3p_amd.solvers import Graph
     reps = max(1, steps // 2)
     out = {}
-    for order in (2, 1):
+    for order, fused in ((2, False), (1, False), (2, True)):
         torch.cuda.synchronize()
         stream = torch.cuda.Stream()       # capture needs a stream other than the default one
         with torch.cuda.stream(stream):
             s.set_clock(getattr(s, "time", 0.0))
             with Graph() as g:
-                s.device_step_pair(order)
+                s.device_step_pair(order, fused)
             settle(g.replay, settle_seconds)
             for _ in range(max(1, warmup // 2)):
                 g.replay()
@@ -286,9 +318,10 @@ def graph_run_minidisk(s, steps, warmup, settle_seconds):
             s.check_errors()
         torch.cuda.synchronize()
         del g
-        out[order] = wall
-    return {"value": s.zones * 2 * reps / out[2], "ms_per_step": 1e3 * out[2] / (2 * reps), "steps": 2 * reps,
-            "forward_euler_ms_per_step": 1e3 * out[1] / (2 * reps),
+        out[(order, fused)] = wall
+    return {"value": s.zones * 2 * reps / out[(2, False)], "ms_per_step": 1e3 * out[(2, False)] / (2 * reps),
+            "steps": 2 * reps, "fused_ms_per_step": 1e3 * out[(2, True)] / (2 * reps),
+            "forward_euler_ms_per_step": 1e3 * out[(1, False)] / (2 * reps),
             "note": "hipGraph replays of two device-clocked RK2 steps (value, ms_per_step) or two forward-Euler "
                     "steps (the reference driver's substep, one stage launch each); the clock kernel computes "
                     "dt and the binary's state on the device"}
@@ -511,6 +544,14 @@ def main():
     }
     if graph:
         result["graph"] = graph
+    if args.workload == "minidisk":
+        fw, fms = fused_run(s, dt_of, args.steps, args.warmup, args.settle_seconds)
+        result["fused"] = {"value": zones * args.steps / fw, "ms_per_step": 1e3 * fw / args.steps,
+                           "launch_ms": fms,
+                           "roofline_frac": 2 * bytes_per_launch / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                           "note": "one launch per RK2 step (both stages, the first stage's state never stored); "
+                                   "roofline_frac: the step's algorithmic bytes (5 nc 8 B per zone) over the "
+                                   "launch duration"}
     if args.workload == "minidisk":
         result["config"] = {"workload": "minidisk iso2d+two_body, 16x16 blocks of 256^2 (16.8M zones), RK2, mixed",
                             "zones": zones, "blocks": 256, "block_size": 256}

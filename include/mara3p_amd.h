@@ -185,6 +185,23 @@ int m3p_minidisk_stage_dev(const double* u, const double* u0, double* out, const
                            int nblk, const double* orbit, const double* dt_dev, double rk_b,
                            const m3p_minidisk_data* data /* host */, int precision, void* stream);
 
+/* One whole RK2 step of the minidisk mesh in one launch (no reference entry
+ * point: the step of control::advance_runge_kutta, src/app_control.hpp:
+ * 169-195, over minidisk::updated_conserved): out = u * rk_b +
+ * L(L(u)) * (1 - rk_b) with the stages at time and time + dt, equal value
+ * for value to m3p_minidisk_stage(u -> u1, time) followed by
+ * m3p_minidisk_stage(u1, u -> out, time + dt), without storing u1.  Uniform
+ * meshes of blocks of 64, 128 or 256 zones a side, M3P_MIXED only, no
+ * viscosity (otherwise M3P_ERR_BAD_ARGUMENT or M3P_ERR_BAD_SHAPE); out must
+ * not alias u.  The _dev form reads the binary's state of both stages from
+ * orbit [12] and dt from dt_dev, as m3p_minidisk_clock writes them. */
+int m3p_minidisk_rk2_step(const double* u, double* out, const int* nbr, const int* bcoord, int nblk,
+                          double time, double dt, double rk_b, const m3p_minidisk_data* data /* host */,
+                          int precision, void* stream);
+int m3p_minidisk_rk2_step_dev(const double* u, double* out, const int* nbr, const int* bcoord, int nblk,
+                              const double* orbit, const double* dt_dev, double rk_b,
+                              const m3p_minidisk_data* data /* host */, int precision, void* stream);
+
 /* 1D Euler stage over nblk blocks of n zones, out-flow at the two ends of the
  * chain described by nbr [nblk][2] = {prev, next} (-1 = outflow boundary). */
 int m3p_euler1d_stage(const double* u, const double* u0, double* out, const int* nbr,

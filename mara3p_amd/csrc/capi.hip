@@ -595,6 +595,46 @@ int m3p_minidisk_stage_dev(const double* u, const double* u0, double* out, const
     return minidisk_stage(u, u0, out, nbr, bcoord, nblk, 0.0, 0.0, orbit, dt_dev, rk_b, data, precision, stream);
 }
 
+static int minidisk_rk2_step(const double* u, double* out, const int* nbr, const int* bcoord, int nblk,
+                             double time, double dt, const double* orbit, const double* dt_dev, double rk_b,
+                             const m3p_minidisk_data* data, int precision, void* stream)
+{
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
+    if (int st = check_minidisk_data(data)) return st;
+    if (!u || !out || !nbr || !bcoord || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad minidisk RK2 step arguments");
+    if (u == out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: RK2 step output must not alias its input");
+    if (precision != M3P_MIXED)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p: the fused RK2 step is compiled for the mixed precision mode only");
+    if (data->kinematic_viscosity != 0.0)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p: the fused RK2 step has no viscous stress (use the stage entry points)");
+    const int n = data->block_size;
+    MinidiskParams p1 = minidisk_params(data, time), p2 = minidisk_params(data, time + dt);
+    p1.orbit = orbit;
+    p2.orbit = orbit ? orbit + 6 : nullptr;
+    StageArgs a{};
+    a.out = out; a.dt = dt; a.dt_dev = dt_dev; a.rk_b = rk_b; a.bcoord = bcoord;
+    a.n = n; a.nblk = nblk; a.err = ds->err;
+    a.dl = 2.0 * data->domain_radius / double(data->block_size) / double(1 << data->depth);
+    return launch_rk2_fused<MinidiskScheme<float>>(p1, p2, MeshLoader<MinidiskScheme<float>>{u, nbr, n}, a,
+                                                   (hipStream_t)stream);
+}
+
+int m3p_minidisk_rk2_step(const double* u, double* out, const int* nbr, const int* bcoord, int nblk,
+                          double time, double dt, double rk_b, const m3p_minidisk_data* data, int precision,
+                          void* stream)
+{
+    return minidisk_rk2_step(u, out, nbr, bcoord, nblk, time, dt, nullptr, nullptr, rk_b, data, precision, stream);
+}
+
+int m3p_minidisk_rk2_step_dev(const double* u, double* out, const int* nbr, const int* bcoord, int nblk,
+                              const double* orbit, const double* dt_dev, double rk_b,
+                              const m3p_minidisk_data* data, int precision, void* stream)
+{
+    if (!orbit || !dt_dev) return fail(M3P_ERR_BAD_ARGUMENT, "m3p_minidisk_rk2_step_dev: orbit and dt_dev are required");
+    return minidisk_rk2_step(u, out, nbr, bcoord, nblk, 0.0, 0.0, orbit, dt_dev, rk_b, data, precision, stream);
+}
+
 int m3p_minidisk_clock(const m3p_minidisk_data* data, double* clock, double* orbit, int rk_order, void* stream)
 {
     StreamState* ds;

@@ -32,6 +32,13 @@ int launch_stage2d_shape(TileShape ts, const typename S::Params& prm, const Ld& 
 template<class S, class Ld>
 int launch_visc(const typename S::Params& prm, const Ld& ld, const StageArgs& a, double nu, hipStream_t s);
 
+// One fused RK2 step (both stages in one launch) over a uniform mesh of
+// blocks of n = 64, 128 or 256 zones a side: p1, p2 the two stages'
+// parameters, a.rk_b the RK weight of the base state.
+template<class S>
+int launch_rk2_fused(const typename S::Params& p1, const typename S::Params& p2, const MeshLoader<S>& ld,
+                     const StageArgs& a, hipStream_t s);
+
 // One 1D work-list stage; cus = compute units of the device (grid sizing).
 template<typename R>
 int launch_stage1d(const EulerParams& prm, const Stage1DArgs& a, hipStream_t s, int cus);

@@ -114,6 +114,8 @@ struct MinidiskScheme {
     // a first stage (no base state) keeps two rows of loads in flight: its
     // stage 1 runs 9 % (mixed) and 2 % (FP64) faster than with one
     static constexpr int kPrefetchRows = 2;
+    // the fused RK2 step (both stages' windows) at four waves per SIMD
+    static constexpr int kFusedWavesPerEU = 3;
     static constexpr bool kCfl = false;
     __device__ static __forceinline__ double cfl_speed(const double*, const MinidiskParams&) { return 0.0; }
     using Real = R;
@@ -159,6 +161,37 @@ struct MinidiskScheme {
             td.xv[t] = v;
             td.xvf[t] = float(v);
             if (t < H) td.xcf[t] = float(0.5 * (v + vertex(prm.R, tc.n, prm.level, tc.bi, tc.k0 + t + 1)));
+        }
+        for (int t = threadIdx.x; t <= TJ; t += blockDim.x) {
+            double v = vertex(prm.R, tc.n, prm.level, tc.bj, tc.c0 + t);
+            td.yv[t] = v;
+            td.yvf[t] = float(v);
+            if (t < TJ) td.ycf[t] = float(0.5 * (v + vertex(prm.R, tc.n, prm.level, tc.bj, tc.c0 + t + 1)));
+        }
+    }
+
+    // TileData of the fused RK2 step, whose tile rows tc.k0 .. tc.k0 + H - 1
+    // reach up to two rows into the blocks before (bi_m) and after (bi_p) on
+    // axis 0: each row takes the centre, and each vertex the coordinate, of
+    // the block that owns it, as that block's own stage computes them
+    // (vertices 0 and n from this block's formula).
+    template<int H, int TJ>
+    __device__ static __forceinline__ void setup_rows(TileData<H, TJ>& td, const Params& prm, const TileCtx& tc,
+                                                      int bi_m, int bi_p)
+    {
+        const int n = tc.n;
+        for (int t = threadIdx.x; t <= H; t += blockDim.x) {
+            const int v = tc.k0 + t;
+            const int ob = v < 0 ? bi_m : (v > n ? bi_p : tc.bi);
+            const int vl = v < 0 ? v + n : (v > n ? v - n : v);
+            const double x = vertex(prm.R, n, prm.level, ob, vl);
+            td.xv[t] = x;
+            td.xvf[t] = float(x);
+            if (t < H) {
+                const int rb = v < 0 ? bi_m : (v >= n ? bi_p : tc.bi);
+                const int rl = v < 0 ? v + n : (v >= n ? v - n : v);
+                td.xcf[t] = float(0.5 * (vertex(prm.R, n, prm.level, rb, rl) + vertex(prm.R, n, prm.level, rb, rl + 1)));
+            }
         }
         for (int t = threadIdx.x; t <= TJ; t += blockDim.x) {
             double v = vertex(prm.R, tc.n, prm.level, tc.bj, tc.c0 + t);

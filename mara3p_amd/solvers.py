@@ -174,23 +174,42 @@ class MinidiskSolver(_StateSolver):
         t, dt = self.clock.cpu().tolist()
         return t, dt
 
-    def device_step(self, u, out, rk_order=2):
+    def rk2_step(self, u, out, time, dt, rk_b=0.5):
+        """One whole RK2 step u -> out in one launch (the fused kernel: the
+        first stage's state is never stored); equal to the two stages."""
+        check(lib().m3p_minidisk_rk2_step(_ptr(u), _ptr(out), _ptr(self.nbr), _ptr(self.bcoord), self.mesh.nblk,
+                                          time, dt, rk_b, ctypes.byref(self.data), self.precision, _stream()))
+
+    def step_fused(self, dt):
+        """step(dt) through the fused RK2 kernel (mixed precision)."""
+        t = self.time
+        self.rk2_step(self.u, self.u2, t, dt)
+        self.u, self.u2 = self.u2, self.u
+        self.time = t * 0.5 + (t + dt) * (1.0 - 0.5)
+
+    def device_step(self, u, out, rk_order=2, fused=False):
         """One step from u into out on the device clock: the clock kernel,
-        then two stages (RK2) or one (forward Euler, the reference driver's
-        substep, problems/minidisk.cpp:172-186)."""
+        then two stages (RK2, or one fused RK2 launch with fused=True) or one
+        stage (forward Euler, the reference driver's substep,
+        problems/minidisk.cpp:172-186)."""
         check(lib().m3p_minidisk_clock(ctypes.byref(self.data), _ptr(self.clock), _ptr(self.orbit), rk_order,
                                        _stream()))
-        if rk_order == 2:
+        if rk_order == 2 and fused:
+            check(lib().m3p_minidisk_rk2_step_dev(_ptr(u), _ptr(out), _ptr(self.nbr), _ptr(self.bcoord),
+                                                  self.mesh.nblk, _ptr(self.orbit),
+                                                  ctypes.c_void_p(self.clock.data_ptr() + 8), 0.5,
+                                                  ctypes.byref(self.data), self.precision, _stream()))
+        elif rk_order == 2:
             self.stage_dev(u, None, self.u1, 0)
             self.stage_dev(self.u1, u, out, 1)
         else:
             self.stage_dev(u, None, out, 0)
 
-    def device_step_pair(self, rk_order=2):
+    def device_step_pair(self, rk_order=2, fused=False):
         """Two device-clocked steps that end in the buffer they began in
         (u -> u2 -> u): the unit a Graph captures."""
-        self.device_step(self.u, self.u2, rk_order)
-        self.device_step(self.u2, self.u, rk_order)
+        self.device_step(self.u, self.u2, rk_order, fused)
+        self.device_step(self.u2, self.u, rk_order, fused)
 
     def autotune(self, reps=5):
         """Pick the fastest compiled tile shape for this mesh (blocking); returns (tj, h, ms)."""
