@@ -4,11 +4,13 @@ Workload (default): the locally isothermal binary minidisk -- [-8, 8]^2 as
 16 x 16 blocks of 256^2 zones (16.8 M zones), equal-mass circular binary
 (a = 1, GM = 1), softening 0.05, Mach 10, sink radius 0.05, Sigma = exp(0.01 z)
 seeded log-normal noise, Keplerian velocities, RK2.  The headline value is the
-mixed kernel (FP32 reconstruction and fluxes, FP64 conserved state); the FP64
-drop-in kernel is timed beside it, and the density L1 difference between the
-two after --l1-steps steps is reported.
+mixed kernel (FP32 reconstruction and fluxes, FP64 conserved state) taking
+each RK2 step in one fused launch (equal value for value to its two stage
+launches, which are timed beside it, or the headline with --two-stage); the
+FP64 drop-in kernel is timed beside it, and the density L1 difference between
+the two after --l1-steps steps is reported.
 
-One "step" = one RK2 step (two fused stage launches) over the whole mesh,
+One "step" = one RK2 step (both stages, in one launch or two) over the whole mesh,
 state resident in HBM.  A zone-update is one zone through one RK2 step
 (the reference's kzps convention, src/app_problem.cpp:84-88).  Before the
 W warmup steps, untimed steps run for --settle-seconds (0.5 s) so that the
@@ -50,6 +52,8 @@ def parse():
                     help="untimed steps for this long before the warmup steps (GPU clocks settle)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--two-stage", action="store_true",
+                    help="minidisk: headline on two stage launches per RK2 step instead of the fused step")
     ap.add_argument("--autotune", action="store_true",
                     help="time every tile shape first and keep the fastest (default: the library's tile table)")
     ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
@@ -109,18 +113,20 @@ CONSERVED = {"minidisk": ["Sigma", "Sx", "Sy"], "kh": ["D", "S1", "S2", "S3", "E
 L1_STEPS = {"kh": 200, "sedov_amr": 100, "ensemble": 100}
 
 
-def l1_report(workload, steps):
+def l1_report(workload, steps, fused=False):
     """Advance the FP64 and the mixed solver through the same time steps and
     report, per conserved variable, mean |mixed - FP64| (absolute and relative
     to mean |FP64|); periodic workloads also report the total mass of each
-    relative to the initial mass."""
+    relative to the initial mass.  fused: the mixed minidisk solver takes the
+    fused RK2 step (equal value for value to its two stage launches)."""
     a, ua, dtf, _ = make_solver(workload, "fp64")
     b, _, _, _ = make_solver(workload, "mixed")
     m0 = math.fsum(ua[:, 0].ravel())
+    bstep = b.step_fused if fused else b.step
     for _ in range(steps):
         dt = dtf(getattr(a, "time", 1.0))
         a.step(dt)
-        b.step(dt)
+        bstep(dt)
     a.check_errors()
     b.check_errors()
     ra, rb = a.state(), b.state()
@@ -134,11 +140,12 @@ def l1_report(workload, steps):
         rep["mass_mixed"] = math.fsum(rb[:, 0].ravel()) / m0
     del a, b
     if workload == "minidisk":
-        rep.update(minidisk_mass_drift(steps))
+        rep.update(minidisk_mass_drift(steps, fused))
+        rep["mixed_step"] = "fused RK2 step" if fused else "two stage launches"
     return rep
 
 
-def minidisk_mass_drift(steps):
+def minidisk_mass_drift(steps, fused=False):
     """Relative drift of the total mass after `steps` RK2 steps of the minidisk
     mesh with the sink and buffer rates at zero (the only terms that add or
     remove mass), in each kernel: |M(t) - M(0)| / M(0)."""
@@ -153,8 +160,9 @@ def minidisk_mass_drift(steps):
     for precision in ("fp64", "mixed"):
         s = MinidiskSolver(data, precision)
         s.set_state(u0)
+        step = s.step_fused if (fused and precision == "mixed") else s.step
         for _ in range(steps):
-            s.step(problems.minidisk_dt(data, s.time))
+            step(problems.minidisk_dt(data, s.time))
         s.check_errors()
         out["mass_drift_" + precision] = abs(math.fsum(s.state()[:, 0].ravel()) - m0) / m0
         del s
@@ -501,10 +509,11 @@ def main():
     # thousands of stage launches bring the GPU to its working clocks before
     # the timed steps (a short timed region right after start-up measured
     # about 10 % slow).
+    fused = args.workload == "minidisk" and not args.two_stage
     rep = None
     if args.l1_steps > 0:
         steps = args.l1_steps if args.workload == "minidisk" else min(args.l1_steps, L1_STEPS[args.workload])
-        rep = l1_report(args.workload, steps)
+        rep = l1_report(args.workload, steps, fused)
     s, u0, dt_of, nc = make_solver(args.workload, "mixed")
     zones = s.zones
     wall, d1, d2 = timed_run(s, dt_of, args.steps, args.warmup, args.autotune, args.settle_seconds)
@@ -521,6 +530,21 @@ def main():
     bytes_per_launch = 0.5 * (2 + 3) * nc * 8 * zones
     kern_ms = 0.5 * (d1 + d2)
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles(args.workload, "mixed")
+    kernel, launch_ms = "stage2d_kernel (two launches per RK2 step)", kern_ms
+    two_stage = None
+    if fused:
+        # the headline: one fused launch per RK2 step, the two stage launches
+        # (equal value for value) reported beside it
+        two_stage = {"value": value, "ms_per_step": ms_per_step, "stage_ms": {"stage1": d1, "stage2": d2},
+                     "tile": tile_in_use(args.workload, "mixed"),
+                     "roofline": {"achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}}
+        fw, fms = fused_run(s, dt_of, args.steps, args.warmup, args.settle_seconds)
+        value = zones * args.steps / fw
+        ms_per_step = 1e3 * fw / args.steps
+        achieved = 2 * bytes_per_launch / (fms * 1e-3) / 1e9
+        traffic = traffic_from_profiles(args.workload, "fused")
+        kernel, launch_ms = "rk2_fused_kernel (one launch per RK2 step)", fms
     result = {
         "metric": "zone-updates/s, FP32-mixed vs FP64 GPU at matched L1 error; % of HBM peak",
         "value": value,
@@ -537,23 +561,21 @@ def main():
         "data": "synthetic",
         "config": {},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles(args.workload, "mixed")},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel, "launch_ms": launch_ms},
         "stage_ms": {"stage1": d1, "stage2": d2},
         "tile": tile,
     }
+    if fused:
+        del result["stage_ms"], result["tile"]
+        result["roofline"]["note"] = ("achieved: the RK2 step's algorithmic bytes (5 nc 8 B per zone: U0, U1 "
+                                      "and U1, U0, U of the two stages) over the fused launch's duration; "
+                                      "traffic: the HBM bytes the fused launch moves (it never stores U1)")
+        result["two_stage"] = two_stage
     if graph:
         result["graph"] = graph
     if args.workload == "minidisk":
-        fw, fms = fused_run(s, dt_of, args.steps, args.warmup, args.settle_seconds)
-        result["fused"] = {"value": zones * args.steps / fw, "ms_per_step": 1e3 * fw / args.steps,
-                           "launch_ms": fms,
-                           "roofline_frac": 2 * bytes_per_launch / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                           "note": "one launch per RK2 step (both stages, the first stage's state never stored); "
-                                   "roofline_frac: the step's algorithmic bytes (5 nc 8 B per zone) over the "
-                                   "launch duration"}
-    if args.workload == "minidisk":
-        result["config"] = {"workload": "minidisk iso2d+two_body, 16x16 blocks of 256^2 (16.8M zones), RK2, mixed",
+        result["config"] = {"workload": "minidisk iso2d+two_body, 16x16 blocks of 256^2 (16.8M zones), RK2, mixed"
+                                        + (", one fused launch per step" if fused else ", two stage launches per step"),
                             "zones": zones, "blocks": 256, "block_size": 256}
     elif args.workload == "kh":
         result["config"] = {"workload": "euler2d Kelvin-Helmholtz 4096^2 periodic (16x16 blocks of 256^2), PLM+HLLC, RK2, CFL 0.4",

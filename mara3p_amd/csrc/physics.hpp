@@ -226,21 +226,25 @@ struct Iso2d {
                     f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) / (ap - am);
             }
         } else {
-            R ul[3] = { pl[0], pl[0] * pl[1], pl[0] * pl[2] };
-            R ur[3] = { pr[0], pr[0] * pr[1], pr[0] * pr[2] };
+            // The same HLL flux with the wave-speed weights folded in:
+            // wp = a+ / (a+ - a-), wm = -a- / (a+ - a-), wpm = -a+ a- / (a+ - a-),
+            // F = wp F_L + wm F_R + wpm (U_L - U_R), and with F = U v_n (+ p on
+            // the normal component), U = Sigma (1, v1, v2):
+            // F_k = Sigma_L v_kL al + Sigma_R v_kR ar (+ cs2 (wp Sigma_L + wm Sigma_R)),
+            // al = wp v_nL + wpm, ar = wm v_nR - wpm (v_0 = 1).  a+ - a- >= c_s > 0.
             R vnl = pl[AXIS], vnr = pr[AXIS];
-            R fl[3] = { vnl * ul[0], vnl * ul[1], vnl * ul[2] };
-            R fr[3] = { vnr * ur[0], vnr * ur[1], vnr * ur[2] };
-            fl[AXIS] = fmaf(pl[0], cs2, fl[AXIS]);
-            fr[AXIS] = fmaf(pr[0], cs2, fr[AXIS]);
             R cs = fsqrt(cs2);
             R ap = fmaxf(0.f, fmaxf(vnl, vnr) + cs);
             R am = fminf(0.f, fminf(vnl, vnr) - cs);
             R inv = frcp(ap - am);
-            R apam = ap * am;
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                f[k] = fmaf(ap, fl[k], fmaf(-am, fr[k], -(ul[k] - ur[k]) * apam)) * inv;
+            R wp = ap * inv, wm = -am * inv;
+            R wpm = wp * -am;
+            R al = fmaf(wp, vnl, wpm), ar = fmaf(wm, vnr, -wpm);
+            R bl = pl[0] * al, br = pr[0] * ar;
+            f[0] = bl + br;
+            f[1] = fmaf(pl[1], bl, pr[1] * br);
+            f[2] = fmaf(pl[2], bl, pr[2] * br);
+            f[AXIS] = fmaf(cs2, fmaf(wp, pl[0], wm * pr[0]), f[AXIS]);
         }
     }
 };
