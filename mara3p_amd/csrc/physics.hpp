@@ -54,9 +54,9 @@ struct Euler {
         }
     }
 
-    // conserved_density :175-183, flux :189-203, outer_wavespeeds :118-128
+    // conserved_density :175-183, flux :189-203
     template<int AXIS>
-    __device__ static __forceinline__ void state(const R* p, R gamma, R igm1, R* u, R* f, R& am, R& ap)
+    __device__ static __forceinline__ void cons_flux(const R* p, R gamma, R igm1, R* u, R* f)
     {
         R d = p[0], v1 = p[1], v2 = p[2], v3 = p[3], pg = p[4];
         if constexpr (is_fp64<R>::value) {
@@ -72,9 +72,6 @@ struct Euler {
             f[2] = vn * u[2] + pg * R(AXIS == 2 ? 1 : 0);
             f[3] = vn * u[3] + pg * R(AXIS == 3 ? 1 : 0);
             f[4] = vn * u[4] + pg * vn;
-            R cs = sqrt(gamma * pg / d);
-            am = vn - cs;
-            ap = vn + cs;
         } else {
             R ls = fmaf(v3, v3, fmaf(v2, v2, v1 * v1));
             u[0] = d;
@@ -89,10 +86,31 @@ struct Euler {
             f[3] = vn * u[3];
             f[AXIS] += pg;
             f[4] = vn * (u[4] + pg);
-            R cs = fsqrt(gamma * pg * frcp(d));
+        }
+    }
+
+    // outer_wavespeeds :118-128
+    template<int AXIS>
+    __device__ static __forceinline__ void speeds(const R* p, R gamma, R& am, R& ap)
+    {
+        if constexpr (is_fp64<R>::value) {
+            R vn = R(0) + p[AXIS];
+            R cs = sqrt(gamma * p[4] / p[0]);
+            am = vn - cs;
+            ap = vn + cs;
+        } else {
+            R vn = p[AXIS];
+            R cs = fsqrt(gamma * p[4] * frcp(p[0]));
             am = vn - cs;
             ap = vn + cs;
         }
+    }
+
+    template<int AXIS>
+    __device__ static __forceinline__ void state(const R* p, R gamma, R igm1, R* u, R* f, R& am, R& ap)
+    {
+        cons_flux<AXIS>(p, gamma, igm1, u, f);
+        speeds<AXIS>(p, gamma, am, ap);
     }
 
     // riemann_hlle :209-222
@@ -121,25 +139,35 @@ struct Euler {
 
     // HLLC (Toro 3rd ed. sec. 10.4), same wave-speed estimates as hlle.  No
     // reference counterpart (the reference's only HLLC is relativistic).
+    // The wave speeds and the contact speed come from the primitives; the
+    // conserved state and flux are then formed for the one side the flux
+    // takes (left if 0 <= S_L, right if S_R <= 0, else the side of the
+    // contact) -- the values of forming both and selecting.
     template<int AXIS>
     __device__ static __forceinline__ void hllc(const R* pl, const R* pr, const Params& prm, R* f)
     {
         R gamma = R(prm.gamma);
         R igm1 = frcp(gamma - R(1));
-        R ul[NC], ur[NC], fl[NC], fr[NC], alm, alp, arm, arp;
-        state<AXIS>(pl, gamma, igm1, ul, fl, alm, alp);
-        state<AXIS>(pr, gamma, igm1, ur, fr, arm, arp);
+        R alm, alp, arm, arp;
+        speeds<AXIS>(pl, gamma, alm, alp);
+        speeds<AXIS>(pr, gamma, arm, arp);
         R sl = smin(alm, arm);
         R sr = smax(alp, arp);
-        if (R(0) <= sl) { for (int k = 0; k < NC; ++k) f[k] = fl[k]; return; }
-        if (sr <= R(0)) { for (int k = 0; k < NC; ++k) f[k] = fr[k]; return; }
         R vl = pl[AXIS], vr = pr[AXIS];
         R ql = pl[0] * (sl - vl), qr = pr[0] * (sr - vr);
+        // unused (and possibly not finite) where the flow is supersonic
         R ss = fdiv(pr[4] - pl[4] + vl * ql - vr * qr, ql - qr);
-        bool left = R(0) <= ss;
-        const R* p = left ? pl : pr;
-        const R* u = left ? ul : ur;
-        const R* fk = left ? fl : fr;
+        const bool sup_l = R(0) <= sl, sup_r = sr <= R(0);
+        const bool left = sup_l || (!sup_r && R(0) <= ss);
+        R p[NC], u[NC], fk[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) p[k] = left ? pl[k] : pr[k];
+        cons_flux<AXIS>(p, gamma, igm1, u, fk);
+        if (sup_l || sup_r) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) f[k] = fk[k];
+            return;
+        }
         R s = left ? sl : sr;
         R vk = p[AXIS];
         R fac = fdiv(p[0] * (s - vk), s - ss);
