@@ -99,17 +99,18 @@ struct Mhd {
         R d = p[0];
         if constexpr (is_fp64<R>::value) {
             // one reciprocal of the density for the three squared speeds,
-            // and bn * bn for pow(bn, 2) (each within an ulp of the
-            // reference's divisions and pow)
+            // bn * bn for pow(bn, 2), and the Newton-refined reciprocal and
+            // square roots (each within a few ulp of the reference's
+            // divisions, pow and sqrt)
             R vn = R(0) + p[1 + a];
-            const R id = R(1) / d;
+            const R id = rcp_nr(d);
             R cs2 = gamma * p[4] * id;
             R ca2 = dot3(b, b) * id;
             R cf2 = cs2 + ca2;
             R bn = R(0) + b[a];
             R can2 = (bn * bn) * id;
-            R cg2 = sqrt(cf2 * cf2 - R(4) * cs2 * can2);
-            R cf = sqrt(R(0.5) * (cf2 + cg2));
+            R cg2 = sqrt_nr(cf2 * cf2 - R(4) * cs2 * can2);
+            R cf = sqrt_nr(R(0.5) * (cf2 + cg2));
             wm = vn - cf;
             wp = vn + cf;
         } else {
@@ -157,7 +158,7 @@ struct Mhd {
         R am = smin(R(0), smin(alm, arm));
         R hi[3];
         if constexpr (is_fp64<R>::value) {
-            const R inv = R(1) / (ap - am);   // one division for the eight components
+            const R inv = rcp_nr(ap - am);   // one reciprocal for the eight components
 #pragma unroll
             for (int k = 0; k < 5; ++k) f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) * inv;
 #pragma unroll
@@ -178,15 +179,27 @@ struct Mhd {
 };
 
 // ---------------------------------------------------------------------------
-// recover_primitive :287-305 (FP64), fastest_speed :248-258 (FP64)
+// recover_primitive :287-305 (FP64), fastest_speed :248-258 (FP64).  NR:
+// one Newton-refined reciprocal of the density in place of the four
+// divisions (within a few ulp; the fused v1 step), else the reference's
+// divisions (the v2 array functions, bitwise).
+template<bool NR = false>
 __device__ __forceinline__ int mhd_recover(const double* u, const double* b, double gamma, double* p)
 {
     double d = u[0];
     double S[3] = { u[1], u[2], u[3] };
-    double ke = 0.5 * Mhd<double>::dot3(S, S) / d;
     double me = 0.5 * Mhd<double>::dot3(b, b);
-    double pg = (u[4] - ke - me) * (gamma - 1);
-    p[0] = d; p[1] = S[0] / d; p[2] = S[1] / d; p[3] = S[2] / d; p[4] = pg;
+    double pg;
+    if constexpr (NR) {
+        const double id = rcp_nr(d);
+        double ke = 0.5 * Mhd<double>::dot3(S, S) * id;
+        pg = (u[4] - ke - me) * (gamma - 1);
+        p[0] = d; p[1] = S[0] * id; p[2] = S[1] * id; p[3] = S[2] * id; p[4] = pg;
+    } else {
+        double ke = 0.5 * Mhd<double>::dot3(S, S) / d;
+        pg = (u[4] - ke - me) * (gamma - 1);
+        p[0] = d; p[1] = S[0] / d; p[2] = S[1] / d; p[3] = S[2] / d; p[4] = pg;
+    }
     p[5] = b[0]; p[6] = b[1]; p[7] = b[2];
     if (d < 0.0) return 1;
     if (pg < 0.0) return 2;
@@ -235,7 +248,7 @@ mhd_prim_kernel(MhdArgs a, R* P)
 #pragma unroll
         for (int q = 0; q < 5; ++q) u[q] = a.uc[q * N3 + c];
         if constexpr (is_fp64<R>::value) {
-            int st = mhd_recover(u, b, a.gamma, p);
+            int st = mhd_recover<true>(u, b, a.gamma, p);
             if (st) report_error(a.err, st == 1 ? 1 : 2, 0, (int)c);
 #pragma unroll
             for (int q = 0; q < 8; ++q) P[q * N3 + c] = p[q];
@@ -527,7 +540,7 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
                 double d3 = double(F3s[q][jl][kl + 1]) - double(F3s[q][jl][kl]);
                 double r;
                 if constexpr (is_fp64<R>::value) {
-                    r = a.uc[q * N3 + c] - (d1 + d2 + d3) * dt / a.dl;
+                    r = a.uc[q * N3 + c] - (d1 + d2 + d3) * dtdl;
                     if (a.u0) r = a.u0[q * N3 + c] * a.rk_b + r * (1.0 - a.rk_b);
                 } else {
                     r = fma(-(d1 + d2 + d3), dtdl, a.uc[q * N3 + c]);
@@ -612,7 +625,7 @@ __device__ __forceinline__ void mhd_update_face(const MhdArgs& a, const FaceE<R>
         b = a.b3[f];
         if (a.u0) b0 = a.b03[f];
     }
-    double r = b - cf * dt / a.dl;
+    double r = b - cf * (dt / a.dl);
     if (a.u0) r = b0 * a.rk_b + r * (1.0 - a.rk_b);
     double* out = A == 1 ? a.bo1 : (A == 2 ? a.bo2 : a.bo3);
     out[f] = r;
@@ -636,7 +649,7 @@ mhd_update_kernel(MhdArgs a, const R* F1, const R* F2, const R* F3, FaceE<R> fe)
         double d1 = double(F1[q * nf + x1]) - double(F1[q * nf + x0]);
         double d2 = double(F2[q * nf + y1]) - double(F2[q * nf + y0]);
         double d3 = double(F3[q * nf + z1]) - double(F3[q * nf + z0]);
-        double r = a.uc[q * N3 + c] - (d1 + d2 + d3) * dt / a.dl;
+        double r = a.uc[q * N3 + c] - (d1 + d2 + d3) * (dt / a.dl);
         if (a.u0) r = a.u0[q * N3 + c] * a.rk_b + r * (1.0 - a.rk_b);
         a.uo[q * N3 + c] = r;
     }

@@ -225,9 +225,10 @@ struct Iso2d {
         }
     }
 
-    // riemann_hlle :156-173 with flux :139-151; RCP (FP64 only) divides once,
-    // by 1 / (a+ - a-) times each component, instead of dividing each
-    // component (within an ulp of the reference's form)
+    // riemann_hlle :156-173 with flux :139-151; RCP (FP64 only, the
+    // minidisk kernel) takes the sound speed and one reciprocal of a+ - a-
+    // from the Newton-refined estimates and multiplies each component by it
+    // (within a few ulp of the reference's form)
     template<int AXIS, bool RCP = false>
     __device__ static __forceinline__ void hlle(const R* pl, const R* pr, R cs2, R* f)
     {
@@ -238,13 +239,13 @@ struct Iso2d {
             R pgl = pl[0] * cs2, pgr = pr[0] * cs2;
             R fl[3] = { vnl * ul[0] + R(0), vnl * ul[1] + pgl * R(AXIS == 1 ? 1 : 0), vnl * ul[2] + pgl * R(AXIS == 2 ? 1 : 0) };
             R fr[3] = { vnr * ur[0] + R(0), vnr * ur[1] + pgr * R(AXIS == 1 ? 1 : 0), vnr * ur[2] + pgr * R(AXIS == 2 ? 1 : 0) };
-            R cs = sqrt(cs2);
+            R cs = RCP ? sqrt_nr(cs2) : sqrt(cs2);
             R alm = vnl - cs, alp = vnl + cs;
             R arm = vnr - cs, arp = vnr + cs;
             R ap = smax(R(0), smax(alp, arp));
             R am = smin(R(0), smin(alm, arm));
             if constexpr (RCP) {
-                const R inv = R(1) / (ap - am);
+                const R inv = rcp_nr(ap - am);
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
                     f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) * inv;

@@ -132,9 +132,9 @@ struct MinidiskScheme {
     __device__ static __forceinline__ bool recover(const double* u, const Params&, R* p)
     {
         if constexpr (is_fp64<R>::value) {
-            // one reciprocal of Sigma (within an ulp of the two divisions of
-            // recover_primitive, src/physics_iso2d.hpp:119-125)
-            const double is = 1.0 / u[0];
+            // one reciprocal of Sigma (within a few ulp of the two divisions
+            // of recover_primitive, src/physics_iso2d.hpp:119-125)
+            const double is = rcp_nr(u[0]);
             p[0] = u[0];
             p[1] = u[1] * is;
             p[2] = u[2] * is;
@@ -233,9 +233,9 @@ struct MinidiskScheme {
                 y = td.yv[fj - tc.c0];
             }
             // -phi / M^2 with phi = -1 / sqrt(r^2 + r_s^2); the division by
-            // M twice is folded into one product (within an ulp)
+            // M twice is folded into one product (within a few ulp)
             double r2 = x * x + y * y;
-            cs2 = (1.0 / sqrt(r2 + prm.s2)) * prm.inv_mach2;
+            cs2 = rsqrt_nr(r2 + prm.s2) * prm.inv_mach2;
             Iso2d<R>::template hlle<AXIS, true>(pl, pr, cs2, f);
             return;
         } else {
@@ -258,14 +258,16 @@ struct MinidiskScheme {
             double x = 0.5 * (td.xv[k - tc.k0] + td.xv[k - tc.k0 + 1]);
             double y = 0.5 * (td.yv[c - tc.c0] + td.yv[c - tc.c0 + 1]);
             // two_body::gravitational_acceleration (src/physics_two_body.hpp:208-218);
-            // pow(q, 1.5) = q sqrt(q) and pow(t, 3) = t t t, within a few ulp
+            // m / pow(q, 1.5) = m rsqrt(q)^3 and pow(t, 3) = t t t, within a few ulp
             double dx1 = x - prm.x1, dy1 = y - prm.y1;
             double dx2 = x - prm.x2, dy2 = y - prm.y2;
             double r21 = dx1 * dx1 + dy1 * dy1;
             double r22 = dx2 * dx2 + dy2 * dy2;
             double q1 = r21 + prm.s2, q2 = r22 + prm.s2;
-            double g1 = -1.0 * prm.m1 / (q1 * sqrt(q1));
-            double g2 = -1.0 * prm.m2 / (q2 * sqrt(q2));
+            // m / q^1.5 = m rsqrt(q)^3
+            const double iq1 = rsqrt_nr(q1), iq2 = rsqrt_nr(q2);
+            double g1 = -1.0 * prm.m1 * (iq1 * iq1 * iq1);
+            double g2 = -1.0 * prm.m2 * (iq2 * iq2 * iq2);
             double ax1 = g1 * dx1, ay1 = g1 * dy1;
             double ax2 = g2 * dx2, ay2 = g2 * dy2;
             // coriolis_term :67-71, centrifugal_term :73-76 (from pc)
@@ -277,23 +279,23 @@ struct MinidiskScheme {
             double t2 = (0.0 + dx2 * dx2) + dy2 * dy2;
             double r61 = t1 * t1 * t1, r62 = t2 * t2 * t2;
             double sr1 = 0.0, sr2 = 0.0;
-            if (r61 < 746.0 * prm.s6) sr1 = prm.sink_rate * exp(-r61 / prm.s6);
-            if (r62 < 746.0 * prm.s6) sr2 = prm.sink_rate * exp(-r62 / prm.s6);
+            if (r61 < 746.0 * prm.s6) sr1 = prm.sink_rate * exp(-r61 * prm.inv_s6);
+            if (r62 < 746.0 * prm.s6) sr2 = prm.sink_rate * exp(-r62 * prm.inv_s6);
             // buffer_rate_field :47-55; 1 + tanh(y) is exactly 0 for y < -20,
             // and then the initial state (initial_conserved_array :182-191)
             // does not enter
-            double r = sqrt((0.0 + x * x) + y * y);
+            double r = sqrt_nr((0.0 + x * x) + y * y);
             double yb = (r - prm.R) * prm.inv_buffer_scale;
             double br = 0.0;
             double ui[3] = { 1.0, 0.0, 0.0 };
             if (yb > -20.0) {
                 // 0.5 (1 + tanh y) = 1 / (1 + exp(-2 y)); reciprocals of the
                 // scale and of r0 in place of divisions (each within an ulp)
-                br = prm.buffer_rate / (1.0 + exp(-2.0 * yb));
-                double ph = -1.0 * 1.0 / sqrt((x * x + y * y) + prm.s2);
-                double r0 = sqrt(x * x + y * y);
-                double vp = sqrt(-ph) - om * r0;
-                const double ir0 = 1.0 / r0;
+                br = prm.buffer_rate * rcp_nr(1.0 + exp(-2.0 * yb));
+                double ph = -1.0 * rsqrt_nr((x * x + y * y) + prm.s2);
+                double r0 = sqrt_nr(x * x + y * y);
+                double vp = sqrt_nr(-ph) - om * r0;
+                const double ir0 = rcp_nr(r0);
                 ui[1] = 1.0 * (vp * (-y * ir0));
                 ui[2] = 1.0 * (vp * (x * ir0));
             }
