@@ -53,10 +53,12 @@ extern "C" {
  * HLLE and HLLC), iso2d / locally isothermal, SRHD and the mhd_scheme_v2
  * array functions equal the reference bit for bit; minidisk and MHD
  * constrained transport agree within 1e-12 relative (minidisk: the device
- * exp and the reciprocal forms of its sources differ from glibc and from the
- * reference's divisions by an ulp or so; MHD: reciprocal forms in the wave
- * speeds and the HLLE flux).  M3P_MIXED computes primitives, slopes and
- * fluxes in FP32 and keeps the conserved state and flux differences in FP64. */
+ * exp, reciprocal forms of its sources and Newton-refined hardware
+ * reciprocals and square roots differ from glibc and from the reference's
+ * divisions by a few ulp; MHD: the same forms in the wave speeds, the HLLE
+ * flux and the primitive recovery).  M3P_MIXED computes primitives, slopes
+ * and fluxes in FP32 and keeps the conserved state and flux differences in
+ * FP64. */
 #define M3P_FP64   0
 #define M3P_MIXED  1
 
