@@ -40,9 +40,10 @@ struct Euler {
             return !(D < 0.0);
         } else {
             // Kinetic energy is removed in FP64 so the pressure keeps its
-            // digits when E is dominated by kinetic energy.
+            // digits when E is dominated by kinetic energy (the reciprocal
+            // of D Newton-refined, within a few ulp).
             double D = u[0];
-            double iD = 1.0 / D;
+            double iD = rcp_nr(D);
             float fiD = float(iD);
             double ls = (u[1] * u[1] + u[2] * u[2]) + u[3] * u[3];
             p[0] = float(D);
@@ -129,11 +130,14 @@ struct Euler {
             for (int k = 0; k < NC; ++k)
                 f[k] = (ap * fl[k] - am * fr[k] - (ul[k] - ur[k]) * ap * am) / (ap - am);
         } else {
+            // the wave-speed weights folded in: F = wp F_L + wm F_R + wpm (U_L - U_R)
+            // with wp = a+ / (a+ - a-), wm = -a- / (a+ - a-), wpm = -a+ a- / (a+ - a-)
             R inv = frcp(ap - am);
-            R apam = ap * am;
+            R wp = ap * inv, wm = -am * inv;
+            R wpm = wp * -am;
 #pragma unroll
             for (int k = 0; k < NC; ++k)
-                f[k] = fmaf(ap, fl[k], fmaf(-am, fr[k], -(ul[k] - ur[k]) * apam)) * inv;
+                f[k] = fmaf(wp, fl[k], fmaf(wm, fr[k], wpm * (ul[k] - ur[k])));
         }
     }
 
