@@ -452,6 +452,31 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
     }
     const double dtdl = dt / a.dl;
 
+    // The plane's two-cell halo ring (NH cells): position of ring cell r.
+    constexpr int NH = 4 * EK + 4 * TJ;
+    auto ring = [&] (int r, int& rj, int& rk) {
+        if (r < 2 * EK)      { rj = r / EK; rk = r % EK; }
+        else if (r < 4 * EK) { rj = TJ + 2 + (r - 2 * EK) / EK; rk = (r - 2 * EK) % EK; }
+        else {
+            const int s2 = r - 4 * EK;
+            rj = 2 + s2 / 4;
+            const int side = s2 % 4;
+            rk = side < 2 ? side : TK + side;
+        }
+    };
+    // Mixed, where every ring cell has a thread of its own: the ring of the
+    // next plane is fetched while this plane computes (its load latency off
+    // the plane's critical path).
+    constexpr bool kRingAhead = NH <= NT && !is_fp64<R>::value;
+    int hj = 0, hk = 0;
+    R Hn[8];
+    if constexpr (kRingAhead) {
+        if (t < NH) {
+            ring(t, hj, hk);
+            loadP(cell(i0, j0 - 2 + hj, k0 - 2 + hk), Hn);
+        }
+    }
+
     for (int il = 0; il < TI; ++il) {
         const int i = i0 + il;
         R Pn[8];
@@ -460,19 +485,20 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
         // LDS plane i: interior from the window, halo ring from memory
 #pragma unroll
         for (int q = 0; q < 8; ++q) Pl[q][jl + 2][kl + 2] = P0[q];
-        for (int r = t; r < 4 * EK + 4 * TJ; r += NT) {
-            int rj, rk;
-            if (r < 2 * EK)      { rj = r / EK; rk = r % EK; }
-            else if (r < 4 * EK) { rj = TJ + 2 + (r - 2 * EK) / EK; rk = (r - 2 * EK) % EK; }
-            else {
-                const int s2 = r - 4 * EK;
-                rj = 2 + s2 / 4;
-                const int side = s2 % 4;
-                rk = side < 2 ? side : TK + side;
-            }
-            const long c = cell(i, j0 - 2 + rj, k0 - 2 + rk);
+        if constexpr (kRingAhead) {
+            if (t < NH) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) Pl[q][rj][rk] = P[q * N3 + c];
+                for (int q = 0; q < 8; ++q) Pl[q][hj][hk] = Hn[q];
+                if (il + 1 < TI) loadP(cell(i + 1, j0 - 2 + hj, k0 - 2 + hk), Hn);
+            }
+        } else {
+            for (int r = t; r < NH; r += NT) {
+                int rj, rk;
+                ring(r, rj, rk);
+                const long c = cell(i, j0 - 2 + rj, k0 - 2 + rk);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) Pl[q][rj][rk] = P[q * N3 + c];
+            }
         }
 
         // x face above the cell (face i + 1)
