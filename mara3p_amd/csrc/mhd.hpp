@@ -464,10 +464,10 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
             rk = side < 2 ? side : TK + side;
         }
     };
-    // Mixed, where every ring cell has a thread of its own: the ring of the
-    // next plane is fetched while this plane computes (its load latency off
-    // the plane's critical path).
-    constexpr bool kRingAhead = NH <= NT && !is_fp64<R>::value;
+    // Where every ring cell has a thread of its own (the 8 x 32 tiles): the
+    // ring of the next plane is fetched while this plane computes (its load
+    // latency off the plane's critical path; 127 VGPRs mixed, 254 FP64).
+    constexpr bool kRingAhead = NH <= NT;
     int hj = 0, hk = 0;
     R Hn[8];
     if constexpr (kRingAhead) {
