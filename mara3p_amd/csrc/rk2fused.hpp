@@ -72,14 +72,15 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
     tc.bi = a.bcoord[2 * b]; tc.bj = a.bcoord[2 * b + 1];
 
     __shared__ typename S::template TileData<HE, TJ> td;   // rows k0 - 2 .. k0 + H + 1
-    __shared__ R e1[7][NC][HE];    // U0 primitives at columns -2, -1, 0, n - 2, n - 1, n, n + 1
-    __shared__ R g1e[NC][HE];      // first stage: half slope at column -1
+    __shared__ R e1[6][NC][HE];    // U0 primitives at columns -2, -1, n - 2, n - 1, n, n + 1
     __shared__ R f1e[NC][HE];      // first stage: y flux at face n
-    __shared__ R e2[7][NC][H];     // U1 primitives at the same columns, rows k0 .. k0 + H - 1
-    __shared__ R g2e[NC][H];
+    __shared__ R e2[6][NC][H];     // U1 primitives at the same columns, rows k0 .. k0 + H - 1
     __shared__ R f2e[NC][H];
-    __shared__ R rowP1[NC][TJ + 2], rowG1[NC][TJ + 1], rowF1[NC][TJ + 1];
-    __shared__ R rowP2[NC][TJ + 2], rowG2[NC][TJ + 1], rowF2[NC][TJ + 1];
+    // a step's row of each stage: rowP [t + 2] = column t, [0] and [1] =
+    // columns -2 and -1, [TJ + 2] = column n (pre-pass); rowF [t] = y flux at
+    // face t, [TJ] = face n (pre-pass)
+    __shared__ R rowP1[NC][TJ + 3], rowF1[NC][TJ + 1];
+    __shared__ R rowP2[NC][TJ + 3], rowF2[NC][TJ + 1];
     // U1 of the last three first-stage rows (row j in slot s % 3), each lane
     // its own column: the second stage reads row j - 3 from the slot the
     // first stage then overwrites, in program order within the lane
@@ -89,12 +90,12 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
     const R theta = R(S::theta(p1));
     bool ok = true;
     auto edge_col = [&] (int q) {
-        return q == 0 ? -2 : q == 1 ? -1 : q == 2 ? 0 : q == 3 ? n - 2 : q == 4 ? n - 1 : q == 5 ? n : n + 1;
+        return q == 0 ? -2 : q == 1 ? -1 : n - 4 + q;
     };
 
     // ---- pre-pass: U0 primitives at the edge columns of rows k0-2 .. k0+H+1
-    for (int idx = t; idx < 7 * HE; idx += TJ) {
-        const int r = idx / 7, q = idx % 7;
+    for (int idx = t; idx < 6 * HE; idx += TJ) {
+        const int r = idx / 6, q = idx % 6;
         double uu[NC];
         R p[NC];
         cons_of(zone_in(b, k0 - 2 + r, edge_col(q)), uu);
@@ -104,8 +105,8 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
     }
     // ---- pre-pass: first stage of the zones at the edge columns of rows
     // k0 .. k0+H-1, one zone per lane (the march's arithmetic), as primitives
-    for (int idx = t; idx < 7 * H; idx += TJ) {
-        const int r = idx / 7, q = idx % 7;
+    for (int idx = t; idx < 6 * H; idx += TJ) {
+        const int r = idx / 6, q = idx % 6;
         const int rl = k0 + r;
         int Z = b, cl = edge_col(q);
         if (cl < 0)       { Z = ym; cl += n; }
@@ -173,152 +174,125 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
         for (int m = 0; m < NC; ++m) e2[q][m][r] = p[m];
     }
     __syncthreads();
-    // edge slopes (column -1) and edge fluxes (face n) of both stages
+    // edge fluxes (face n) of both stages
     for (int r = t; r < HE + H; r += TJ) {
         const bool first = r < HE;
         const int rr = first ? r : r - HE;
-        R v[7][NC];
+        R v[4][NC];
 #pragma unroll
-        for (int q = 0; q < 7; ++q)
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
-            for (int m = 0; m < NC; ++m) v[q][m] = first ? e1[q][m][rr] : e2[q][m][rr];
-        R pl[NC], pr[NC], f[NC], g[NC];
+            for (int m = 0; m < NC; ++m) v[q][m] = first ? e1[q + 2][m][rr] : e2[q + 2][m][rr];
+        R pl[NC], pr[NC], f[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-            g[m] = plm_half(v[0][m], v[1][m], v[2][m], theta);
-            const R gA = plm_half(v[3][m], v[4][m], v[5][m], theta);
-            const R gB = plm_half(v[4][m], v[5][m], v[6][m], theta);
-            pl[m] = v[4][m] + gA;
-            pr[m] = v[5][m] - gB;
+            const R gA = plm_half(v[0][m], v[1][m], v[2][m], theta);
+            const R gB = plm_half(v[1][m], v[2][m], v[3][m], theta);
+            pl[m] = v[1][m] + gA;
+            pr[m] = v[2][m] - gB;
         }
         if (first) S::template face<2>(pl, pr, p1, td, tc, k0 - 2 + rr, n, f);
         else       S::template face<2>(pl, pr, p2, td, tc, k0 + rr, n, f);
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-            if (first) { g1e[m][rr] = g[m]; f1e[m][rr] = f[m]; }
-            else       { g2e[m][rr] = g[m]; f2e[m][rr] = f[m]; }
+            if (first) f1e[m][rr] = f[m];
+            else       f2e[m][rr] = f[m];
         }
     }
     __syncthreads();
 
-    // ---- first stage: warm-up at row k0 - 2 ------------------------------
-    // Fb1: x flux at the face above row j, kept in FP32 (the update takes
-    // its FP64 value, which is exact); U0 of rows j and k is re-read for the
-    // two updates (a cache hit: row j was fetched three rows earlier), the
-    // loads issued after the second barrier of the step so that they are in
-    // flight during the y fluxes (4 % faster than loading at the updates;
-    // loading at the top of the step needs 175 VGPRs and runs slower)
-    R Pa1[NC], Pb1[NC], ga1[NC], Fb1[NC];
-    double Unx[NC];
-    {
-        R Pm2[NC], Pm1[NC], gm[NC], pl[NC], pr[NC], f[NC];
-        ok &= ld.prim_row(b, k0 - 4, c, p1, Pm2);
-        ok &= ld.prim_row(b, k0 - 3, c, p1, Pm1);
-        ok &= ld.prim_row(b, k0 - 2, c, p1, Pa1);
-        ok &= ld.prim_row(b, k0 - 1, c, p1, Pb1);
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            gm[m] = plm_half(Pm2[m], Pm1[m], Pa1[m], theta);
-            ga1[m] = plm_half(Pm1[m], Pa1[m], Pb1[m], theta);
-            pl[m] = Pm1[m] + gm[m];
-            pr[m] = Pa1[m] - ga1[m];
-        }
-        S::template face<1>(pl, pr, p1, td, tc, k0 - 2, c, Fb1);
-        ld.fetch_row(b, k0, c, Unx);
-    }
-
-    // second-stage window (filled after the first four first-stage rows)
-    R Pa2[NC] = {}, Pb2[NC] = {}, ga2[NC] = {}, Pn2[NC] = {}, Pw0[NC] = {}, Pw1[NC] = {}, Fb2[NC] = {};
+    // ---- the march -------------------------------------------------------
+    // Step s handles first-stage row j = k0 - 2 + s (s < H + 4) and second-
+    // stage row k = j - 3 (s >= 5); the two share the two workgroup barriers
+    // of a step.  Every window of the march is three register sets
+    // used in rotation, ROT = (s - 5) mod 3 (a compile-time constant), so the
+    // steady loop, unrolled by three, shifts no window by moves:
+    //   first stage:  P1[ROT], P1[ROT+1], P1[ROT+2] = primitives of rows j,
+    //                 j + 1, j + 2 (the last recovered in the step); G1[ROT],
+    //                 G1[ROT+1] = half slopes at rows j, j + 1; F1[ROT],
+    //                 F1[ROT+1] = x fluxes at faces j, j + 1; UN[ROT] = U0 of
+    //                 row j + 2, row j + 3 fetched into UN[ROT+1];
+    //   second stage: P2[ROT], P2[ROT+1], P2[ROT+2] = U1 primitives of rows
+    //                 k, k + 1, k + 2; G2, F2 as G1, F1 for rows k, k + 1.
+    // The first stage's new U1 row (j) lands in P2[ROT], the slot of row k,
+    // dead by then.  The x fluxes stay in FP32 (the update takes their FP64
+    // values, which are exact).  U0 of rows j and k is re-read for the two
+    // updates (a cache hit: row j was fetched three rows earlier), issued
+    // after the step's first barrier so that the loads are in flight during
+    // the y fluxes.  Each lane computes the y half slope of its left
+    // neighbour as well as its own, from the same LDS operands, so no barrier
+    // separates the y slopes from the y fluxes.
+    R P1[3][NC], G1[3][NC], F1[3][NC], P2[3][NC], G2[3][NC], F2[3][NC];
+    double UN[3][NC];
     const double wb = 1.0 - a.rk_b;
 
-    // March step s: first-stage row j = k0 - 2 + s (DO1, s < H + 4), second-
-    // stage row k = j - 3 (DO2, s >= 5), sharing the three workgroup barriers
-    // of a row.  One loop body for every step (wave-uniform flags): the
-    // first stage's rows k0 - 2 .. k0 + 1 seed the second stage's warm-up,
-    // done after step 3.  Unrolled by three, most window shifts become
-    // register renames (158 VGPRs, three waves per SIMD, no spills; the U1
-    // queue lives in LDS, which kept 18 registers out of the march).
-#pragma unroll 3
-    for (int s = 0; s <= HE; ++s) {
-        const bool DO1 = s < HE;
-        const bool DO2 = s >= 5;
+    auto step = [&] (auto do1_c, auto do2_c, auto rot_c, auto fetch_c, int s) {
+        constexpr bool DO1 = decltype(do1_c)::value, DO2 = decltype(do2_c)::value;
+        constexpr bool FETCH = decltype(fetch_c)::value;
+        constexpr int A = decltype(rot_c)::value, B = (A + 1) % 3, C = (A + 2) % 3;
+        constexpr int slot = C;                // s % 3, since s = 5 + ROT (mod 3)
         const int j = k0 - 2 + s;
         const int k = j - 3;
         const int kl = k - k0;
-        R Pc1[NC], gb1[NC], Fa1[NC], gy1[NC], Fy1[NC];
-        R gb2[NC], Fa2[NC], gy2[NC], Fy2[NC];
+        R gy1[NC], Fy1[NC], gy2[NC], Fy2[NC];
         double U0j[NC], U0k[NC];
         // phase A: rows ahead, x slopes and x fluxes; LDS rows
-        if (DO1) {
-            double raw[NC];
-#pragma unroll
-            for (int m = 0; m < NC; ++m) raw[m] = Unx[m];
-            if (s + 1 < HE) ld.fetch_row(b, j + 3, c, Unx);
-            ok &= S::recover(raw, p1, Pc1);
+        if constexpr (DO1) {
+            if constexpr (FETCH) ld.fetch_row(b, j + 3, c, UN[B]);
+            ok &= S::recover(UN[A], p1, P1[C]);
             R pl[NC], pr[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                gb1[m] = plm_half(Pa1[m], Pb1[m], Pc1[m], theta);
-                pl[m] = Pa1[m] + ga1[m];
-                pr[m] = Pb1[m] - gb1[m];
+                G1[B][m] = plm_half(P1[A][m], P1[B][m], P1[C][m], theta);
+                pl[m] = P1[A][m] + G1[A][m];
+                pr[m] = P1[B][m] - G1[B][m];
             }
-            S::template face<1>(pl, pr, p1, td, tc, j + 1, c, Fa1);
+            S::template face<1>(pl, pr, p1, td, tc, j + 1, c, F1[B]);
 #pragma unroll
-            for (int m = 0; m < NC; ++m) rowP1[m][t + 1] = Pa1[m];
+            for (int m = 0; m < NC; ++m) rowP1[m][t + 2] = P1[A][m];
             if (t == 0) {
 #pragma unroll
-                for (int m = 0; m < NC; ++m) { rowP1[m][0] = e1[1][m][s]; rowG1[m][0] = g1e[m][s]; }
+                for (int m = 0; m < NC; ++m) { rowP1[m][0] = e1[0][m][s]; rowP1[m][1] = e1[1][m][s]; }
             }
             if (t == TJ - 1) {
 #pragma unroll
-                for (int m = 0; m < NC; ++m) rowP1[m][TJ + 1] = e1[5][m][s];
+                for (int m = 0; m < NC; ++m) rowP1[m][TJ + 2] = e1[4][m][s];
             }
         }
-        if (DO2) {
+        if constexpr (DO2) {
             R pl[NC], pr[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                gb2[m] = plm_half(Pa2[m], Pb2[m], Pn2[m], theta);
-                pl[m] = Pa2[m] + ga2[m];
-                pr[m] = Pb2[m] - gb2[m];
+                G2[B][m] = plm_half(P2[A][m], P2[B][m], P2[C][m], theta);
+                pl[m] = P2[A][m] + G2[A][m];
+                pr[m] = P2[B][m] - G2[B][m];
             }
-            S::template face<1>(pl, pr, p2, td, tc, k + 1, c, Fa2);
+            S::template face<1>(pl, pr, p2, td, tc, k + 1, c, F2[B]);
 #pragma unroll
-            for (int m = 0; m < NC; ++m) rowP2[m][t + 1] = Pa2[m];
+            for (int m = 0; m < NC; ++m) rowP2[m][t + 2] = P2[A][m];
             if (t == 0) {
 #pragma unroll
-                for (int m = 0; m < NC; ++m) { rowP2[m][0] = e2[1][m][kl]; rowG2[m][0] = g2e[m][kl]; }
+                for (int m = 0; m < NC; ++m) { rowP2[m][0] = e2[0][m][kl]; rowP2[m][1] = e2[1][m][kl]; }
             }
             if (t == TJ - 1) {
 #pragma unroll
-                for (int m = 0; m < NC; ++m) rowP2[m][TJ + 1] = e2[5][m][kl];
+                for (int m = 0; m < NC; ++m) rowP2[m][TJ + 2] = e2[4][m][kl];
             }
         }
         __syncthreads();
-        // phase B: y half slopes
-        if (DO1) {
-#pragma unroll
-            for (int m = 0; m < NC; ++m) {
-                gy1[m] = plm_half(rowP1[m][t], Pa1[m], rowP1[m][t + 2], theta);
-                rowG1[m][t + 1] = gy1[m];
-            }
-        }
-        if (DO2) {
-#pragma unroll
-            for (int m = 0; m < NC; ++m) {
-                gy2[m] = plm_half(rowP2[m][t], Pa2[m], rowP2[m][t + 2], theta);
-                rowG2[m][t + 1] = gy2[m];
-            }
-        }
-        __syncthreads();
-        // the RK base rows of the two updates, in flight during phase C
-        if (DO1) cons_of(ld.row(b, j) + c, U0j);
-        if (DO2) cons_of(ld.row(b, k) + c, U0k);
-        // phase C: y fluxes at the lower face of each column
-        if (DO1) {
+        if constexpr (DO1) cons_of(ld.row(b, j) + c, U0j);
+        if constexpr (DO2) cons_of(ld.row(b, k) + c, U0k);
+        // phase B: y half slopes (own column and the left neighbour's), y
+        // fluxes at the lower face of each column
+        if constexpr (DO1) {
             R pl[NC], pr[NC];
 #pragma unroll
-            for (int m = 0; m < NC; ++m) { pl[m] = rowP1[m][t] + rowG1[m][t]; pr[m] = Pa1[m] - gy1[m]; }
+            for (int m = 0; m < NC; ++m) {
+                gy1[m] = plm_half(rowP1[m][t + 1], P1[A][m], rowP1[m][t + 3], theta);
+                const R gl = plm_half(rowP1[m][t], rowP1[m][t + 1], P1[A][m], theta);
+                pl[m] = rowP1[m][t + 1] + gl;
+                pr[m] = P1[A][m] - gy1[m];
+            }
             S::template face<2>(pl, pr, p1, td, tc, j, c, Fy1);
 #pragma unroll
             for (int m = 0; m < NC; ++m) rowF1[m][t] = Fy1[m];
@@ -327,10 +301,15 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
                 for (int m = 0; m < NC; ++m) rowF1[m][TJ] = f1e[m][s];
             }
         }
-        if (DO2) {
+        if constexpr (DO2) {
             R pl[NC], pr[NC];
 #pragma unroll
-            for (int m = 0; m < NC; ++m) { pl[m] = rowP2[m][t] + rowG2[m][t]; pr[m] = Pa2[m] - gy2[m]; }
+            for (int m = 0; m < NC; ++m) {
+                gy2[m] = plm_half(rowP2[m][t + 1], P2[A][m], rowP2[m][t + 3], theta);
+                const R gl = plm_half(rowP2[m][t], rowP2[m][t + 1], P2[A][m], theta);
+                pl[m] = rowP2[m][t + 1] + gl;
+                pr[m] = P2[A][m] - gy2[m];
+            }
             S::template face<2>(pl, pr, p2, td, tc, k, c, Fy2);
 #pragma unroll
             for (int m = 0; m < NC; ++m) rowF2[m][t] = Fy2[m];
@@ -340,18 +319,17 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
             }
         }
         __syncthreads();
-        // phase D: updates
-        if (DO2) {
-            double dfx[NC], dfy[NC], res[NC];
+        // phase C: updates; the second stage reads row k's U1 from the queue
+        // slot before the first stage puts row j there
+        if constexpr (DO2) {
+            double dfx[NC], dfy[NC], res[NC], u1k[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                dfx[m] = double(Fa2[m]) - double(Fb2[m]);
+                dfx[m] = double(F2[B][m]) - double(F2[A][m]);
                 dfy[m] = double(rowF2[m][t + 1]) - double(Fy2[m]);
+                u1k[m] = u1q[slot][m][t];
             }
-            double u1k[NC];
-#pragma unroll
-            for (int m = 0; m < NC; ++m) u1k[m] = u1q[s % 3][m][t];
-            S::update(p2, td, tc, k, c, u1k, Pa2, dfx, dfy, dt, dl, res);
+            S::update(p2, td, tc, k, c, u1k, P2[A], dfx, dfy, dt, dl, res);
             double* o = a.out + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
@@ -360,49 +338,91 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
                 else                             wv = fma(U0k[m], a.rk_b, res[m] * wb);
                 __builtin_nontemporal_store(wv, &o[(size_t)m * n * n]);
             }
-#pragma unroll
-            for (int m = 0; m < NC; ++m) { Pa2[m] = Pb2[m]; Pb2[m] = Pn2[m]; ga2[m] = gb2[m]; Fb2[m] = Fa2[m]; }
         }
-        if (DO1) {
+        if constexpr (DO1) {
             double dfx[NC], dfy[NC], u1[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                dfx[m] = double(Fa1[m]) - double(Fb1[m]);
+                dfx[m] = double(F1[B][m]) - double(F1[A][m]);
                 dfy[m] = double(rowF1[m][t + 1]) - double(Fy1[m]);
             }
-            S::update(p1, td, tc, j, c, U0j, Pa1, dfx, dfy, dt, dl, u1);
-            // U1 row j: its primitives feed the second stage's next row, its
-            // conserved state the second stage's update three rows later
-            ok &= S::recover(u1, p2, Pn2);
+            S::update(p1, td, tc, j, c, U0j, P1[A], dfx, dfy, dt, dl, u1);
+            ok &= S::recover(u1, p2, P2[A]);
 #pragma unroll
-            for (int m = 0; m < NC; ++m) {
-                u1q[s % 3][m][t] = u1[m];
-                Pa1[m] = Pb1[m]; Pb1[m] = Pc1[m]; ga1[m] = gb1[m]; Fb1[m] = Fa1[m];
-            }
+            for (int m = 0; m < NC; ++m) u1q[slot][m][t] = u1[m];
         }
-        if (s == 0) {
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    using R0 = std::integral_constant<int, 0>;
+    using R1 = std::integral_constant<int, 1>;
+    using R2 = std::integral_constant<int, 2>;
+
+    // first-stage warm-up at row k0 - 2, into the slots of step 0 (ROT 1)
+    {
+        R Pm2[NC], Pm1[NC], gm[NC], pl[NC], pr[NC];
+        ok &= ld.prim_row(b, k0 - 4, c, p1, Pm2);
+        ok &= ld.prim_row(b, k0 - 3, c, p1, Pm1);
+        ok &= ld.prim_row(b, k0 - 2, c, p1, P1[1]);
+        ok &= ld.prim_row(b, k0 - 1, c, p1, P1[2]);
 #pragma unroll
-            for (int m = 0; m < NC; ++m) Pw0[m] = Pn2[m];
-        } else if (s == 1) {
-#pragma unroll
-            for (int m = 0; m < NC; ++m) Pw1[m] = Pn2[m];
-        } else if (s == 2) {
-#pragma unroll
-            for (int m = 0; m < NC; ++m) Pa2[m] = Pn2[m];
-        } else if (s == 3) {
-            // second-stage warm-up: slopes at k0 - 1 and k0, flux at the face k0
-            R gm[NC], pl[NC], pr[NC];
-#pragma unroll
-            for (int m = 0; m < NC; ++m) {
-                Pb2[m] = Pn2[m];
-                gm[m] = plm_half(Pw0[m], Pw1[m], Pa2[m], theta);
-                ga2[m] = plm_half(Pw1[m], Pa2[m], Pb2[m], theta);
-                pl[m] = Pw1[m] + gm[m];
-                pr[m] = Pa2[m] - ga2[m];
-            }
-            S::template face<1>(pl, pr, p2, td, tc, k0, c, Fb2);
+        for (int m = 0; m < NC; ++m) {
+            gm[m] = plm_half(Pm2[m], Pm1[m], P1[1][m], theta);
+            G1[1][m] = plm_half(Pm1[m], P1[1][m], P1[2][m], theta);
+            pl[m] = Pm1[m] + gm[m];
+            pr[m] = P1[1][m] - G1[1][m];
         }
+        S::template face<1>(pl, pr, p1, td, tc, k0 - 2, c, F1[1]);
+        ld.fetch_row(b, k0, c, UN[1]);
     }
+    // steps 0 .. 4: first stage only; the U1 rows k0 - 2 and k0 - 1 (steps
+    // 0 and 1) seed the second stage's warm-up, rows k0 .. k0 + 2 (steps
+    // 2 .. 4) land in P2[0..2], the slots of step 5 (ROT 0)
+    R Pw0[NC], Pw1[NC];
+    step(T_{}, F_{}, R1{}, T_{}, 0);
+#pragma unroll
+    for (int m = 0; m < NC; ++m) Pw0[m] = P2[1][m];
+    step(T_{}, F_{}, R2{}, T_{}, 1);
+#pragma unroll
+    for (int m = 0; m < NC; ++m) Pw1[m] = P2[2][m];
+    step(T_{}, F_{}, R0{}, T_{}, 2);
+    step(T_{}, F_{}, R1{}, T_{}, 3);
+    {
+        // second-stage warm-up: slopes at k0 - 1 and k0, flux at the face k0
+        R gm[NC], pl[NC], pr[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            gm[m] = plm_half(Pw0[m], Pw1[m], P2[0][m], theta);
+            G2[0][m] = plm_half(Pw1[m], P2[0][m], P2[1][m], theta);
+            pl[m] = Pw1[m] + gm[m];
+            pr[m] = P2[0][m] - G2[0][m];
+        }
+        S::template face<1>(pl, pr, p2, td, tc, k0, c, F2[0]);
+    }
+    step(T_{}, F_{}, R2{}, T_{}, 4);
+    // steady march, both stages; the last first-stage step (s = H + 3)
+    // fetches nothing
+    constexpr int NS = HE - 5;                 // steps with both stages
+    constexpr int NL = (NS - 1) / 3;           // unrolled iterations
+    constexpr int NR = NS - 3 * NL;            // 1 .. 3 steps after them
+    int s = 5;
+    for (int it = 0; it < NL; ++it, s += 3) {
+        step(T_{}, T_{}, R0{}, T_{}, s);
+        step(T_{}, T_{}, R1{}, T_{}, s + 1);
+        step(T_{}, T_{}, R2{}, T_{}, s + 2);
+    }
+    if constexpr (NR == 1) {
+        step(T_{}, T_{}, R0{}, F_{}, s);
+    } else if constexpr (NR == 2) {
+        step(T_{}, T_{}, R0{}, T_{}, s);
+        step(T_{}, T_{}, R1{}, F_{}, s + 1);
+    } else {
+        step(T_{}, T_{}, R0{}, T_{}, s);
+        step(T_{}, T_{}, R1{}, T_{}, s + 1);
+        step(T_{}, T_{}, R2{}, F_{}, s + 2);
+    }
+    // the second stage's last row
+    step(F_{}, T_{}, std::integral_constant<int, NR % 3>{}, F_{}, HE);
     if (!ok) report_error(a.err, 1, b, 0);
 }
 

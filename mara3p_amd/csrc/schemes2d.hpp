@@ -114,7 +114,7 @@ struct MinidiskScheme {
     // a first stage (no base state) keeps two rows of loads in flight: its
     // stage 1 runs 9 % (mixed) and 2 % (FP64) faster than with one
     static constexpr int kPrefetchRows = 2;
-    // the fused RK2 step (both stages' windows) at four waves per SIMD
+    // the fused RK2 step (both stages' windows) at three waves per SIMD
     static constexpr int kFusedWavesPerEU = 3;
     static constexpr bool kCfl = false;
     __device__ static __forceinline__ double cfl_speed(const double*, const MinidiskParams&) { return 0.0; }
