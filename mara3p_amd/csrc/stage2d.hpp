@@ -300,13 +300,21 @@ __device__ __forceinline__ void store_edge_flux(double* reg, int b, int e, int n
     for (int m = 0; m < NC; ++m) reg[(((size_t)b * 4 + e) * NC + m) * n + i] = double(f[m]);
 }
 
+// Occupancy floor of a stage kernel (waves per SIMD, which bounds its VGPRs):
+// the scheme's kWavesPerEU, or for a first stage its kWavesPerEUFirst where
+// the scheme has one.
+template<class S, bool FIRST, class = void>
+struct StageWaves { static constexpr int value = S::kWavesPerEU; };
+template<class S>
+struct StageWaves<S, true, std::void_t<decltype(S::kWavesPerEUFirst)>> { static constexpr int value = S::kWavesPerEUFirst; };
+
 // KEEPU: keep each zone's conserved state in a three-row register window from
 // its load to its update (no second read of the state); otherwise re-load it
 // at the update (fewer registers).  FIRST: a stage without a base state
 // (a.u0 == nullptr is then assumed); it has fewer loads in flight per row, so
 // a scheme may ask it to fetch more rows ahead (S::kPrefetchRows).
 template<class S, class Ld, int H, int TJ, bool KEEPU, bool FIRST = false>
-__global__ void __launch_bounds__(TJ) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU, 8)))
+__global__ void __launch_bounds__(TJ) __attribute__((amdgpu_waves_per_eu(StageWaves<S, FIRST>::value, 8)))
 stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
 {
     constexpr int NC = S::NC;
