@@ -24,6 +24,7 @@ template<typename R>
 struct EulerScheme {
     static constexpr int NC = 5;
     static constexpr int kWavesPerEU = is_fp64<R>::value ? 2 : 3;   // occupancy floor (VGPR budget)
+    static constexpr bool kLateBase = false;  // RK base loaded at the top of a row (pipelined loader)
     static constexpr int kMarchUnroll = 2;   // rows per march iteration with a pipelined loader
     // rows a first stage fetches ahead (pipelined loader): two made stage 1
     // 2 % faster for KH FP64 and the quad-tree mixed stage, 1 % slower for
@@ -107,11 +108,11 @@ __device__ __forceinline__ void resolve_params(MinidiskParams& p)
 template<typename R>
 struct MinidiskScheme {
     static constexpr int NC = 3;
-    static constexpr int kWavesPerEU = is_fp64<R>::value ? 2 : 4;
-    // the FP64 first stage at three waves per SIMD (168 VGPRs, 3 spilled):
-    // 13 % faster than at two; the second stage spills 24 there and runs
-    // 26 % slower
-    static constexpr int kWavesPerEUFirst = is_fp64<R>::value ? 3 : 4;
+    // FP64 at three waves per SIMD (168 VGPRs): the first stage spills 3
+    // and runs 13 % faster than at two; the second, with its RK base loaded
+    // at the update (kLateBase), spills none and runs 5 % faster
+    static constexpr int kWavesPerEU = is_fp64<R>::value ? 3 : 4;
+    static constexpr bool kLateBase = is_fp64<R>::value;
     // three rows per iteration rotate the three-row window in registers; the
     // FP64 kernel runs faster at two (register pressure)
     static constexpr int kMarchUnroll = is_fp64<R>::value ? 2 : 3;
@@ -381,6 +382,7 @@ template<typename R>
 struct IsoTableScheme {
     static constexpr int NC = 3;
     static constexpr int kWavesPerEU = 2;
+    static constexpr bool kLateBase = false;
     static constexpr int kMarchUnroll = 2;
     static constexpr int kPrefetchRows = 1;
     static constexpr bool kCfl = false;

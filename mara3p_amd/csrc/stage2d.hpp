@@ -300,21 +300,13 @@ __device__ __forceinline__ void store_edge_flux(double* reg, int b, int e, int n
     for (int m = 0; m < NC; ++m) reg[(((size_t)b * 4 + e) * NC + m) * n + i] = double(f[m]);
 }
 
-// Occupancy floor of a stage kernel (waves per SIMD, which bounds its VGPRs):
-// the scheme's kWavesPerEU, or for a first stage its kWavesPerEUFirst where
-// the scheme has one.
-template<class S, bool FIRST, class = void>
-struct StageWaves { static constexpr int value = S::kWavesPerEU; };
-template<class S>
-struct StageWaves<S, true, std::void_t<decltype(S::kWavesPerEUFirst)>> { static constexpr int value = S::kWavesPerEUFirst; };
-
 // KEEPU: keep each zone's conserved state in a three-row register window from
 // its load to its update (no second read of the state); otherwise re-load it
 // at the update (fewer registers).  FIRST: a stage without a base state
 // (a.u0 == nullptr is then assumed); it has fewer loads in flight per row, so
 // a scheme may ask it to fetch more rows ahead (S::kPrefetchRows).
 template<class S, class Ld, int H, int TJ, bool KEEPU, bool FIRST = false>
-__global__ void __launch_bounds__(TJ) __attribute__((amdgpu_waves_per_eu(StageWaves<S, FIRST>::value, 8)))
+__global__ void __launch_bounds__(TJ) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU, 8)))
 stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
 {
     constexpr int NC = S::NC;
@@ -399,7 +391,9 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
     // through a selected address (no divergent branch).  Loaders that
     // support it are software-pipelined: the raw state of the row three
     // ahead and the RK base of the current row are fetched at the top of an
-    // iteration, so their latency overlaps the iteration's arithmetic.
+    // iteration, so their latency overlaps the iteration's arithmetic (a
+    // scheme with kLateBase loads the RK base at the update instead, holding
+    // fewer registers across the row).
     auto march = [&] (auto dir) {
         constexpr int D = decltype(dir)::value;
         const int kfirst = D > 0 ? tc.k0 : tc.k0 + H - 1;
@@ -468,10 +462,12 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
                     for (int m = 0; m < NC; ++m) Unx[i][m] = Unx[i + 1][m];
                 // the last row a tile reads is kfirst + (H + 1) D
                 if (kk + kAhead < H) ld.fetch_row(b, k + (2 + kAhead) * D, c, Unx[kAhead - 1]);
-                if (!FIRST && a.u0) {
-                    const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
+                if constexpr (!S::kLateBase) {
+                    if (!FIRST && a.u0) {
+                        const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
-                    for (int m = 0; m < NC; ++m) U0[m] = __builtin_nontemporal_load(&z0[(size_t)m * n * n]);
+                        for (int m = 0; m < NC; ++m) U0[m] = __builtin_nontemporal_load(&z0[(size_t)m * n * n]);
+                    }
                 }
                 ok &= ld.finish_row(b, k + 2 * D, c, prm, raw, Uc, Pc);
             } else {
@@ -550,7 +546,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             double* o = a.out + ((size_t)b * NC * n + k) * n + c;
             double wv[NC];
             if (!FIRST && a.u0) {
-                if constexpr (!Ld::kPrefetch) {
+                if constexpr (!Ld::kPrefetch || S::kLateBase) {
                     const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
                     for (int m = 0; m < NC; ++m) U0[m] = __builtin_nontemporal_load(&z0[(size_t)m * n * n]);
