@@ -19,7 +19,8 @@ K timed steps see the GPU at its working clocks.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload minidisk|kh|sedov_amr|ensemble|mhd]
 
 The other workloads are timed the same way for the record: kh = euler2d
-Kelvin-Helmholtz 4096^2 with HLLC; sedov_amr = euler2d Sedov blast on a 2:1 quad-tree
+Kelvin-Helmholtz 4096^2 with HLLC (headline on the fused RK2 step as well,
+the two stage launches beside it); sedov_amr = euler2d Sedov blast on a 2:1 quad-tree
 of 1408 leaves of 64^2 (levels 3-6); ensemble = 512 independent 1D Riemann
 problems of 512-8192 zones in one work-list launch per stage; mhd = ideal
 MHD with constrained transport, Orszag-Tang at 256^3 (the stage time step
@@ -53,7 +54,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--two-stage", action="store_true",
-                    help="minidisk: headline on two stage launches per RK2 step instead of the fused step")
+                    help="minidisk, kh: headline on two stage launches per RK2 step instead of the fused step")
     ap.add_argument("--autotune", action="store_true",
                     help="time every tile shape first and keep the fastest (default: the library's tile table)")
     ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
@@ -117,8 +118,8 @@ def l1_report(workload, steps, fused=False):
     """Advance the FP64 and the mixed solver through the same time steps and
     report, per conserved variable, mean |mixed - FP64| (absolute and relative
     to mean |FP64|); periodic workloads also report the total mass of each
-    relative to the initial mass.  fused: the mixed minidisk solver takes the
-    fused RK2 step (equal value for value to its two stage launches)."""
+    relative to the initial mass.  fused: the mixed solver takes the fused
+    RK2 step (equal value for value to its two stage launches)."""
     a, ua, dtf, _ = make_solver(workload, "fp64")
     b, _, _, _ = make_solver(workload, "mixed")
     m0 = math.fsum(ua[:, 0].ravel())
@@ -230,16 +231,21 @@ def timed_run(s, dt_of, steps, warmup, autotune=False, settle_seconds=0.0):
 
 
 # ---------------------------------------------------------------------------
-def graph_run(s, steps, warmup, cfl=0.4):
+def graph_run(s, steps, warmup, cfl=0.4, rk2=False):
     """The stages captured in a hipGraph: one replay is two CFL-limited RK2
     steps.  Fused: each step's second stage also reduces the next step's CFL
     time step (the library's *_stage_cfl entry points); separate: a CFL
-    reduction pass before each step.  Whole-step wall time over the replays."""
+    reduction pass before each step; rk2 (kh): each step one fused RK2
+    launch that also reduces the next step's time step.  Whole-step wall
+    time over the replays."""
     import torch
-    from mara3p_amd.solvers import Graph, cfl_step_pair, cfl_step_pair_fused
+    from mara3p_amd.solvers import Graph, cfl_step_pair, cfl_step_pair_fused, cfl_step_pair_rk2
     reps = max(1, steps // 2)
     out = {}
-    for name, pair in (("fused", cfl_step_pair_fused), ("separate", cfl_step_pair)):
+    pairs = [("fused", cfl_step_pair_fused), ("separate", cfl_step_pair)]
+    if rk2:
+        pairs.append(("rk2", cfl_step_pair_rk2))
+    for name, pair in pairs:
         torch.cuda.synchronize()
         stream = torch.cuda.Stream()       # capture needs a stream other than the default one
         with torch.cuda.stream(stream):
@@ -258,10 +264,14 @@ def graph_run(s, steps, warmup, cfl=0.4):
         torch.cuda.synchronize()
         del g
         out[name] = wall
-    return {"value": s.zones * 2 * reps / out["fused"], "ms_per_step": 1e3 * out["fused"] / (2 * reps),
-            "separate_cfl_ms_per_step": 1e3 * out["separate"] / (2 * reps), "steps": 2 * reps,
-            "note": "hipGraph replays of two CFL-limited RK2 steps; value and ms_per_step with the CFL reduction "
-                    "fused into each step's second stage, separate_cfl_ms_per_step with a reduction pass per step"}
+    res = {"value": s.zones * 2 * reps / out["fused"], "ms_per_step": 1e3 * out["fused"] / (2 * reps),
+           "separate_cfl_ms_per_step": 1e3 * out["separate"] / (2 * reps), "steps": 2 * reps,
+           "note": "hipGraph replays of two CFL-limited RK2 steps; value and ms_per_step with the CFL reduction "
+                   "fused into each step's second stage, separate_cfl_ms_per_step with a reduction pass per step"}
+    if rk2:
+        res["rk2_fused_ms_per_step"] = 1e3 * out["rk2"] / (2 * reps)
+        res["note"] += ", rk2_fused_ms_per_step with each step one fused RK2 launch reducing the next time step"
+    return res
 
 
 def fused_run(s, dt_of, steps, warmup, settle_seconds):
@@ -269,9 +279,10 @@ def fused_run(s, dt_of, steps, warmup, settle_seconds):
     stage's state never stored): wall time and the mean launch duration from
     HIP events on the launch stream."""
     import torch
-    settle(lambda: s.step_fused(dt_of(s.time)), settle_seconds)
+    t_of = lambda: getattr(s, "time", 0.0)
+    settle(lambda: s.step_fused(dt_of(t_of())), settle_seconds)
     for _ in range(warmup):
-        s.step_fused(dt_of(s.time))
+        s.step_fused(dt_of(t_of()))
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
@@ -288,7 +299,7 @@ def fused_run(s, dt_of, steps, warmup, settle_seconds):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        s.step_fused(dt_of(s.time))
+        s.step_fused(dt_of(t_of()))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     s.rk2_step = orig
@@ -509,7 +520,7 @@ def main():
     # thousands of stage launches bring the GPU to its working clocks before
     # the timed steps (a short timed region right after start-up measured
     # about 10 % slow).
-    fused = args.workload == "minidisk" and not args.two_stage
+    fused = args.workload in ("minidisk", "kh") and not args.two_stage
     rep = None
     if args.l1_steps > 0:
         steps = args.l1_steps if args.workload == "minidisk" else min(args.l1_steps, L1_STEPS[args.workload])
@@ -521,7 +532,7 @@ def main():
     graph = None
     if not args.no_graph:
         if args.workload in ("kh", "sedov_amr", "ensemble"):
-            graph = graph_run(s, args.steps, args.warmup)
+            graph = graph_run(s, args.steps, args.warmup, rk2=args.workload == "kh")
         elif args.workload == "minidisk":
             graph = graph_run_minidisk(s, args.steps, args.warmup, args.settle_seconds)
     ms_per_step = 1e3 * wall / args.steps
@@ -578,7 +589,9 @@ def main():
                                         + (", one fused launch per step" if fused else ", two stage launches per step"),
                             "zones": zones, "blocks": 256, "block_size": 256}
     elif args.workload == "kh":
-        result["config"] = {"workload": "euler2d Kelvin-Helmholtz 4096^2 periodic (16x16 blocks of 256^2), PLM+HLLC, RK2, CFL 0.4",
+        result["config"] = {"workload": "euler2d Kelvin-Helmholtz 4096^2 periodic (16x16 blocks of 256^2), PLM+HLLC, RK2, "
+                                        "CFL 0.4, mixed"
+                                        + (", one fused launch per step" if fused else ", two stage launches per step"),
                             "zones": zones, "blocks": 256, "block_size": 256}
     elif args.workload == "sedov_amr":
         result["config"] = {"workload": "euler2d Sedov blast on a 2:1 quad-tree, 1408 leaves of 64^2 (levels 3-6), "

@@ -381,6 +381,23 @@ int m3p_amr_euler2d_stage_conservative(const double* u, const double* u0, double
 int m3p_euler_cfl_dt_blocks(const double* u, int nblk, long zones_per_block, const double* dl_blk,
                             double gamma_law_index, double cfl, double* dt_out, void* stream);
 
+/* One whole RK2 step of a uniform euler2d mesh in one launch (no reference
+ * entry point: the step of euler2d::updated_solution,
+ * src/module_euler.cpp:304-323, without storing its intermediate state):
+ * out = u * rk_b + L(L(u)) * (1 - rk_b), equal value for value to
+ * m3p_euler2d_stage(u -> u1) followed by m3p_euler2d_stage(u1, u -> out)
+ * with the same dt (or dt_dev), dl, theta, gamma and hllc.  Blocks of 64,
+ * 128 or 256 zones a side, M3P_MIXED only (otherwise M3P_ERR_BAD_ARGUMENT
+ * or M3P_ERR_BAD_SHAPE); out must not alias u.  The _cfl form also writes
+ * the time step of `out` to dt_next (device double), as
+ * m3p_euler2d_stage_cfl does for its output, bit for bit. */
+int m3p_euler2d_rk2_step(const double* u, double* out, const int* nbr, int nblk, int n, double dt,
+                         const double* dt_dev, double dl, double rk_b, double plm_theta, double gamma_law_index,
+                         int hllc, int precision, void* stream);
+int m3p_euler2d_rk2_step_cfl(const double* u, double* out, const int* nbr, int nblk, int n, double dt,
+                             const double* dt_dev, double dl, double rk_b, double plm_theta, double gamma_law_index,
+                             int hllc, double cfl, double* dt_next, int precision, void* stream);
+
 /* The three Euler stages above with the CFL reduction of their output fused
  * in (no reference counterpart: a device step driver's CFL step without a
  * separate pass over the state).  Same arguments and results as

@@ -635,6 +635,32 @@ int m3p_minidisk_rk2_step_dev(const double* u, double* out, const int* nbr, cons
     return minidisk_rk2_step(u, out, nbr, bcoord, nblk, 0.0, 0.0, orbit, dt_dev, rk_b, data, precision, stream);
 }
 
+static int euler2d_rk2_step(const double* u, double* out, const int* nbr, int nblk, int n, double dt,
+                            const double* dt_dev, double dl, double rk_b, double plm_theta, double gamma_law_index,
+                            int hllc, int precision, void* stream, unsigned long long* rate_slot)
+{
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
+    if (!u || !out || !nbr || nblk <= 0) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: bad euler2d RK2 step arguments");
+    if (u == out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: RK2 step output must not alias its input");
+    if (precision != M3P_MIXED)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p: the fused RK2 step is compiled for the mixed precision mode only");
+    EulerParams prm{ gamma_law_index, plm_theta, hllc };
+    StageArgs a{};
+    a.out = out; a.dt = dt; a.dt_dev = dt_dev; a.dl = dl; a.rk_b = rk_b;
+    a.n = n; a.nblk = nblk; a.err = ds->err; a.rate_slot = rate_slot;
+    return launch_rk2_fused<EulerScheme<float>>(prm, prm, MeshLoader<EulerScheme<float>>{u, nbr, n}, a,
+                                                (hipStream_t)stream);
+}
+
+int m3p_euler2d_rk2_step(const double* u, double* out, const int* nbr, int nblk, int n, double dt,
+                         const double* dt_dev, double dl, double rk_b, double plm_theta, double gamma_law_index,
+                         int hllc, int precision, void* stream)
+{
+    return euler2d_rk2_step(u, out, nbr, nblk, n, dt, dt_dev, dl, rk_b, plm_theta, gamma_law_index, hllc, precision,
+                            stream, nullptr);
+}
+
 int m3p_minidisk_clock(const m3p_minidisk_data* data, double* clock, double* orbit, int rk_order, void* stream)
 {
     StreamState* ds;
@@ -969,6 +995,16 @@ int m3p_euler2d_stage_cfl(const double* u, const double* u0, double* out, const 
     return cfl_stage(cfl, dt_next, stream, [&] (unsigned long long* slot) {
         return euler2d_stage(u, u0, out, nbr, nblk, n, dt, dt_dev, dl, rk_b, plm_theta, gamma_law_index, hllc,
                              precision, stream, slot);
+    });
+}
+
+int m3p_euler2d_rk2_step_cfl(const double* u, double* out, const int* nbr, int nblk, int n, double dt,
+                             const double* dt_dev, double dl, double rk_b, double plm_theta, double gamma_law_index,
+                             int hllc, double cfl, double* dt_next, int precision, void* stream)
+{
+    return cfl_stage(cfl, dt_next, stream, [&] (unsigned long long* slot) {
+        return euler2d_rk2_step(u, out, nbr, nblk, n, dt, dt_dev, dl, rk_b, plm_theta, gamma_law_index, hllc,
+                                precision, stream, slot);
     });
 }
 

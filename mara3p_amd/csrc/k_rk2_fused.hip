@@ -1,4 +1,5 @@
-// Fused RK2 step of MinidiskScheme<float> (its own unit: a large kernel).
+// Fused RK2 steps of the mixed minidisk and euler2d schemes (their own unit:
+// large kernels).
 #include "launch.hpp"
 #include "rk2fused.hpp"
 
@@ -28,5 +29,9 @@ template int launch_rk2_fused<MinidiskScheme<float>>(const MinidiskScheme<float>
                                                       const MinidiskScheme<float>::Params&,
                                                       const MeshLoader<MinidiskScheme<float>>&, const StageArgs&,
                                                       hipStream_t);
+template int launch_rk2_fused<EulerScheme<float>>(const EulerScheme<float>::Params&,
+                                                   const EulerScheme<float>::Params&,
+                                                   const MeshLoader<EulerScheme<float>>&, const StageArgs&,
+                                                   hipStream_t);
 
 } // namespace m3p

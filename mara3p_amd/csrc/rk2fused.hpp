@@ -15,7 +15,11 @@
 // time, with the arithmetic of the march).  Every zone's U1 is computed with
 // the operands, geometry and order of the two-stage kernels, so the step
 // equals two stage2d launches value for value; HBM sees U0 (plus the halo
-// rows) and U once instead of U0, U1, U1, U0 and U.
+// rows) and U once instead of U0, U1, U1, U0 and U.  Instantiated for the
+// minidisk scheme (problems/minidisk_scheme.cpp:294-355 per stage) and the
+// euler2d scheme (euler2d::updated_solution, src/module_euler.cpp:304-323);
+// a scheme with a fused CFL reduction (kCfl) also reduces the CFL rate of U
+// in the same launch, as its second-stage kernel does.
 //
 // Zones outside the block are read the way the block that owns their row
 // reads them in its own stage: rows through the axis-0 neighbours, then
@@ -67,9 +71,15 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
         for (int m = 0; m < NC; ++m) uu[m] = z[(size_t)m * n * n];
     };
 
+    // block coordinates (schemes with geometry); without them every block
+    // takes (bi0, bj0)
+    auto bi_of = [&] (int Z) { return a.bcoord ? a.bcoord[2 * Z] : a.bi0; };
+    auto bj_of = [&] (int Z) { return a.bcoord ? a.bcoord[2 * Z + 1] : a.bj0; };
     TileCtx tc;
     tc.b = b; tc.n = n; tc.k0 = k0 - 2; tc.c0 = 0;
-    tc.bi = a.bcoord[2 * b]; tc.bj = a.bcoord[2 * b + 1];
+    tc.bi = bi_of(b); tc.bj = bj_of(b);
+    double rate = 0.0;                                    // fused CFL reduction (S::kCfl)
+    const double idl = a.rate_slot ? 1.0 / dl : 0.0;
 
     __shared__ typename S::template TileData<HE, TJ> td;   // rows k0 - 2 .. k0 + H + 1
     __shared__ R e1[6][NC][HE];    // U0 primitives at columns -2, -1, n - 2, n - 1, n, n + 1
@@ -86,7 +96,7 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
     // first stage then overwrites, in program order within the lane
     __shared__ double u1q[3][NC][TJ];
 
-    S::setup_rows(td, p1, tc, a.bcoord[2 * xm], a.bcoord[2 * xp]);
+    S::setup_rows(td, p1, tc, bi_of(xm), bi_of(xp));
     const R theta = R(S::theta(p1));
     bool ok = true;
     auto edge_col = [&] (int q) {
@@ -113,7 +123,7 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
         else if (cl >= n) { Z = yp; cl -= n; }
         TileCtx zc;
         zc.b = Z; zc.n = n; zc.k0 = rl; zc.c0 = cl;
-        zc.bi = a.bcoord[2 * Z]; zc.bj = a.bcoord[2 * Z + 1];
+        zc.bi = bi_of(Z); zc.bj = bj_of(Z);
         typename S::template TileData<1, 1> zt;
         S::setup_zone(zt, p1, zc);
         R P[5][NC], Q[5][NC];
@@ -331,12 +341,15 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
             }
             S::update(p2, td, tc, k, c, u1k, P2[A], dfx, dfy, dt, dl, res);
             double* o = a.out + ((size_t)b * NC * n + k) * n + c;
+            double wv[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                double wv;
-                if constexpr (is_fp64<R>::value) wv = U0k[m] * a.rk_b + res[m] * wb;
-                else                             wv = fma(U0k[m], a.rk_b, res[m] * wb);
-                __builtin_nontemporal_store(wv, &o[(size_t)m * n * n]);
+                if constexpr (is_fp64<R>::value) wv[m] = U0k[m] * a.rk_b + res[m] * wb;
+                else                             wv[m] = fma(U0k[m], a.rk_b, res[m] * wb);
+                __builtin_nontemporal_store(wv[m], &o[(size_t)m * n * n]);
+            }
+            if constexpr (S::kCfl) {
+                if (a.rate_slot) rate = fmax(rate, S::cfl_speed(wv, p2) * idl);
             }
         }
         if constexpr (DO1) {
@@ -424,6 +437,9 @@ rk2_fused_kernel(typename S::Params p1, typename S::Params p2, MeshLoader<S> ld,
     // the second stage's last row
     step(F_{}, T_{}, std::integral_constant<int, NR % 3>{}, F_{}, HE);
     if (!ok) report_error(a.err, 1, b, 0);
+    if constexpr (S::kCfl) {
+        if (a.rate_slot) workgroup_atomic_max<TJ>(rate, a.rate_slot);
+    }
 }
 
 } // namespace m3p

@@ -31,6 +31,9 @@ struct EulerScheme {
     // the quad-tree FP64 stage
     static constexpr int kPrefetchRows = 2;
     static constexpr bool kCfl = true;       // stage kernels can fuse the CFL reduction
+    // the fused RK2 step (both stages' windows of five components) at two
+    // waves per SIMD: 178 VGPRs, and its 61 KB of LDS fit two workgroups per CU
+    static constexpr int kFusedWavesPerEU = 2;
     using Real = R;
     using Params = EulerParams;
     __device__ static __forceinline__ double cfl_speed(const double* u, const Params& prm)
@@ -46,6 +49,10 @@ struct EulerScheme {
     }
     template<int H, int TJ>
     __device__ static __forceinline__ void setup(TileData<H, TJ>&, const Params&, const TileCtx&) {}
+    // no geometry: the fused RK2 step's tile rows and per-zone pre-pass
+    template<int H, int TJ>
+    __device__ static __forceinline__ void setup_rows(TileData<H, TJ>&, const Params&, const TileCtx&, int, int) {}
+    __device__ static __forceinline__ void setup_zone(TileData<1, 1>&, const Params&, const TileCtx&) {}
 
     template<int AXIS, int H, int TJ>
     __device__ static __forceinline__ void face(const R* pl, const R* pr, const Params& prm,

@@ -109,6 +109,25 @@ class Euler2DSolver(_StateSolver):
         self.stage(self.u1, self.u, self.u2, dt)
         self.u, self.u2 = self.u2, self.u
 
+    def rk2_step(self, u, out, dt, rk_b=0.5, dt_dev=None, cfl=None, dt_next=None):
+        """One whole RK2 step u -> out in one launch (the fused kernel, mixed
+        precision: the first stage's state is never stored); equal to the two
+        stages.  With cfl and dt_next the CFL time step of `out` is reduced in
+        the same launch."""
+        if cfl is not None:
+            check(lib().m3p_euler2d_rk2_step_cfl(_ptr(u), _ptr(out), _ptr(self.nbr), self.mesh.nblk, self.mesh.n,
+                                                 dt, _ptr(dt_dev), self.dl, rk_b, self.theta, self.gamma, self.hllc,
+                                                 cfl, _ptr(dt_next), self.precision, _stream()))
+            return
+        check(lib().m3p_euler2d_rk2_step(_ptr(u), _ptr(out), _ptr(self.nbr), self.mesh.nblk, self.mesh.n, dt,
+                                         _ptr(dt_dev), self.dl, rk_b, self.theta, self.gamma, self.hllc,
+                                         self.precision, _stream()))
+
+    def step_fused(self, dt):
+        """step(dt) through the fused RK2 kernel (mixed precision)."""
+        self.rk2_step(self.u, self.u2, dt)
+        self.u, self.u2 = self.u2, self.u
+
     def cfl_dt(self, cfl):
         """CFL dt from the current state into self.dt_dev (device) and returned."""
         zpb = self.mesh.n * self.mesh.n
@@ -427,6 +446,16 @@ def cfl_step_pair_fused(solver, cfl):
     for a, b in ((u, u2), (u2, u)):
         solver.stage(a, None, u1, 0.0, dt_dev=dt)
         solver.stage(u1, a, b, 0.0, dt_dev=dt, cfl=cfl, dt_next=dt)
+
+
+def cfl_step_pair_rk2(solver, cfl):
+    """cfl_step_pair_fused with each step one fused RK2 launch (an
+    Euler2DSolver in mixed precision): the launch also reduces the next
+    step's CFL time step into solver.dt_dev.  Same results as cfl_step_pair."""
+    u, u2 = solver.u, solver.u2
+    dt = solver.dt_dev
+    for a, b in ((u, u2), (u2, u)):
+        solver.rk2_step(a, b, 0.0, dt_dev=dt, cfl=cfl, dt_next=dt)
 
 
 class MhdSolver:
