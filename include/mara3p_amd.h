@@ -398,6 +398,24 @@ int m3p_euler2d_rk2_step_cfl(const double* u, double* out, const int* nbr, int n
                              const double* dt_dev, double dl, double rk_b, double plm_theta, double gamma_law_index,
                              int hllc, double cfl, double* dt_next, int precision, void* stream);
 
+/* One whole RK2 step of the 1D work-list in one launch (no reference entry
+ * point: two euler1d::updated_conserved passes, src/module_euler.cpp:107-135,
+ * and the RK combine, without storing the intermediate state): equal value
+ * for value to m3p_euler1d_stage_blocks(u -> u1) followed by
+ * m3p_euler1d_stage_blocks(u1, u -> out) with the same dt (or dt_dev),
+ * dl_blk, theta and gamma.  The prev/next links of nbr must be mutual where
+ * both blocks exist (chains and rings; otherwise the error latch receives
+ * M3P_ERR_BAD_SHAPE); blocks of 16, 32, 64, 128 or a multiple of 256 zones,
+ * M3P_MIXED only (otherwise M3P_ERR_BAD_ARGUMENT or M3P_ERR_BAD_SHAPE); out
+ * must not alias u.  The _cfl form also writes the time step of `out` to
+ * dt_next (device double), as m3p_euler1d_stage_blocks_cfl does. */
+int m3p_euler1d_rk2_step_blocks(const double* u, double* out, const int* nbr, const double* dl_blk, int nblk, int n,
+                                double dt, const double* dt_dev, double rk_b, double plm_theta,
+                                double gamma_law_index, int precision, void* stream);
+int m3p_euler1d_rk2_step_blocks_cfl(const double* u, double* out, const int* nbr, const double* dl_blk, int nblk,
+                                    int n, double dt, const double* dt_dev, double rk_b, double plm_theta,
+                                    double gamma_law_index, double cfl, double* dt_next, int precision, void* stream);
+
 /* The three Euler stages above with the CFL reduction of their output fused
  * in (no reference counterpart: a device step driver's CFL step without a
  * separate pass over the state).  Same arguments and results as

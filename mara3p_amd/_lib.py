@@ -103,6 +103,8 @@ SIGNATURES = {
     "m3p_minidisk_rk2_step_dev": (_I, [_P, _P, _P, _P, _I, _P, _P, _D, ctypes.POINTER(MinidiskData), _I, _P]),
     "m3p_euler2d_rk2_step": (_I, [_P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _D, _I, _I, _P]),
     "m3p_euler2d_rk2_step_cfl": (_I, [_P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _D, _I, _D, _P, _I, _P]),
+    "m3p_euler1d_rk2_step_blocks": (_I, [_P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _I, _P]),
+    "m3p_euler1d_rk2_step_blocks_cfl": (_I, [_P, _P, _P, _P, _I, _I, _D, _P, _D, _D, _D, _D, _P, _I, _P]),
     "m3p_euler1d_stage": (_I, [_P, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _I, _P]),
     "m3p_euler_cfl_dt": (_I, [_P, _I, _L, _D, _D, _D, _P, _P]),
     "m3p_set_stage2d_tile": (_I, [_I, _I, _I, _I]),

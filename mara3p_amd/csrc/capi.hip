@@ -915,6 +915,32 @@ int m3p_euler1d_stage_blocks(const double* u, const double* u0, double* out, con
                                 precision, stream, nullptr);
 }
 
+static int euler1d_rk2_step_blocks(const double* u, double* out, const int* nbr, const double* dl_blk, int nblk,
+                                   int n, double dt, const double* dt_dev, double rk_b, double plm_theta,
+                                   double gamma_law_index, int precision, void* stream, unsigned long long* rate_slot)
+{
+    StreamState* ds;
+    if (int st = stream_state(stream, &ds)) return st;
+    if (!u || !out || !nbr || !dl_blk || nblk <= 0)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p_euler1d_rk2_step_blocks: bad arguments");
+    if (u == out) return fail(M3P_ERR_BAD_ARGUMENT, "m3p: RK2 step output must not alias its input");
+    if (precision != M3P_MIXED)
+        return fail(M3P_ERR_BAD_ARGUMENT, "m3p: the fused RK2 step is compiled for the mixed precision mode only");
+    EulerParams prm{ gamma_law_index, plm_theta, 0 };
+    Stage1DArgs a{};
+    a.u = u; a.out = out; a.nbr = nbr; a.dt = dt; a.dt_dev = dt_dev; a.dl_blk = dl_blk; a.rk_b = rk_b;
+    a.n = n; a.nblk = nblk; a.err = ds->err; a.rate_slot = rate_slot;
+    return launch_rk2_fused_1d(prm, a, (hipStream_t)stream);
+}
+
+int m3p_euler1d_rk2_step_blocks(const double* u, double* out, const int* nbr, const double* dl_blk, int nblk, int n,
+                                double dt, const double* dt_dev, double rk_b, double plm_theta,
+                                double gamma_law_index, int precision, void* stream)
+{
+    return euler1d_rk2_step_blocks(u, out, nbr, dl_blk, nblk, n, dt, dt_dev, rk_b, plm_theta, gamma_law_index,
+                                   precision, stream, nullptr);
+}
+
 static int amr_euler2d_stage(const double* u, const double* u0, double* out, double* pe_work, const int* recipes,
                              const double* dl_leaf, int nleaf, int n, double dt, const double* dt_dev, double rk_b,
                              double plm_theta, double gamma_law_index, int hllc, int precision, void* stream,
@@ -1005,6 +1031,16 @@ int m3p_euler2d_rk2_step_cfl(const double* u, double* out, const int* nbr, int n
     return cfl_stage(cfl, dt_next, stream, [&] (unsigned long long* slot) {
         return euler2d_rk2_step(u, out, nbr, nblk, n, dt, dt_dev, dl, rk_b, plm_theta, gamma_law_index, hllc,
                                 precision, stream, slot);
+    });
+}
+
+int m3p_euler1d_rk2_step_blocks_cfl(const double* u, double* out, const int* nbr, const double* dl_blk, int nblk,
+                                    int n, double dt, const double* dt_dev, double rk_b, double plm_theta,
+                                    double gamma_law_index, double cfl, double* dt_next, int precision, void* stream)
+{
+    return cfl_stage(cfl, dt_next, stream, [&] (unsigned long long* slot) {
+        return euler1d_rk2_step_blocks(u, out, nbr, dl_blk, nblk, n, dt, dt_dev, rk_b, plm_theta, gamma_law_index,
+                                       precision, stream, slot);
     });
 }
 

@@ -124,9 +124,18 @@ __device__ __forceinline__ void report_error(int* err, int code, int block, int 
 // of 64, or one partial wave), then one atomic max into one of kRateSlots
 // slots picked by the workgroup index (non-negative doubles order like their
 // bit patterns; spreading the atomics keeps thousands of workgroups that end
-// together from serialising on one address).  Every thread of the workgroup
-// must call it.
+// together from serialising on one address).  The atomic is skipped when the
+// slot already holds at least the workgroup's maximum: slots only grow, so a
+// stale read can only cost an atomic, never lose one, and a launch of tens
+// of thousands of small workgroups does few atomics after its first ones.
+// Every thread of the workgroup must call it.
 constexpr int kRateSlots = 64;
+
+__device__ __forceinline__ void slot_atomic_max(unsigned long long* slot, double m)
+{
+    const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
+    if (mb > __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMax(slot, mb);
+}
 
 template<int NT>
 __device__ __forceinline__ void workgroup_atomic_max(double v, unsigned long long* slots)
@@ -141,7 +150,7 @@ __device__ __forceinline__ void workgroup_atomic_max(double v, unsigned long lon
         if (threadIdx.x == 0) {
             double m = part[0];
             for (int w = 1; w < NT / 64; ++w) m = fmax(m, part[w]);
-            atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+            slot_atomic_max(slot, m);
         }
     } else {
         // a partial wave: no shuffles across inactive lanes
@@ -150,7 +159,7 @@ __device__ __forceinline__ void workgroup_atomic_max(double v, unsigned long lon
         if (threadIdx.x == 0) {
             double m = lanes[0];
             for (int i = 1; i < NT; ++i) m = fmax(m, lanes[i]);
-            atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+            slot_atomic_max(slot, m);
         }
     }
 }

@@ -43,6 +43,11 @@ int launch_rk2_fused(const typename S::Params& p1, const typename S::Params& p2,
 template<typename R>
 int launch_stage1d(const EulerParams& prm, const Stage1DArgs& a, hipStream_t s, int cus);
 
+// One fused RK2 step (both stages in one launch, mixed precision) over the
+// 1D work-list: a.u the state, a.out the result, a.rk_b the RK weight of the
+// base state; blocks of 16, 32, 64, 128 or a multiple of 256 zones.
+int launch_rk2_fused_1d(const EulerParams& prm, const Stage1DArgs& a, hipStream_t s);
+
 // One RK stage of MHD constrained transport; work = workspace of
 // mhd_workspace_bytes(N, precision) bytes.
 long mhd_workspace_bytes(int N, int precision);

@@ -363,6 +363,7 @@ class Euler1DEnsembleSolver(_StateSolver):
         self.nbr = torch.as_tensor(mesh.neighbours(), device=self.device)
         self.dl = torch.as_tensor(mesh.cell_spacing(), dtype=torch.float64, device=self.device)
         self.dt_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self._mutual = self.links_mutual()
 
     def stage(self, u, u0, out, dt, rk_b=0.5, dt_dev=None, cfl=None, dt_next=None):
         if cfl is not None:
@@ -378,6 +379,40 @@ class Euler1DEnsembleSolver(_StateSolver):
         self.stage(self.u, None, self.u1, dt)
         self.stage(self.u1, self.u, self.u2, dt)
         self.u, self.u2 = self.u2, self.u
+
+    def rk2_step(self, u, out, dt, rk_b=0.5, dt_dev=None, cfl=None, dt_next=None):
+        """One whole RK2 step u -> out in one launch (the fused kernel, mixed
+        precision: the first stage's state is never stored); equal to the two
+        stages.  With cfl and dt_next the CFL time step of `out` is reduced in
+        the same launch."""
+        if not self._mutual:
+            raise ValueError("the fused RK2 step needs mutual prev/next links (a one-block problem has none); "
+                             "use step()")
+        if cfl is not None:
+            check(lib().m3p_euler1d_rk2_step_blocks_cfl(_ptr(u), _ptr(out), _ptr(self.nbr), _ptr(self.dl),
+                                                        self.mesh.nblk, self.mesh.n, dt, _ptr(dt_dev), rk_b,
+                                                        self.theta, self.gamma, cfl, _ptr(dt_next), self.precision,
+                                                        _stream()))
+            return
+        check(lib().m3p_euler1d_rk2_step_blocks(_ptr(u), _ptr(out), _ptr(self.nbr), _ptr(self.dl), self.mesh.nblk,
+                                                self.mesh.n, dt, _ptr(dt_dev), rk_b, self.theta, self.gamma,
+                                                self.precision, _stream()))
+
+    def step_fused(self, dt):
+        """step(dt) through the fused RK2 kernel (mixed precision)."""
+        self.rk2_step(self.u, self.u2, dt)
+        self.u, self.u2 = self.u2, self.u
+
+    def links_mutual(self):
+        """True when every prev/next link of the chain table is returned by
+        the linked block (what the fused RK2 step needs; a one-block problem,
+        whose next link is itself and whose prev end is outflow, is not)."""
+        nbr = self.mesh.neighbours()
+        b = np.arange(nbr.shape[0])
+        prev, nxt = nbr[:, 0], nbr[:, 1]
+        ok_prev = (prev < 0) | (nbr[np.maximum(prev, 0), 1] == b)
+        ok_next = (nxt < 0) | (nbr[np.maximum(nxt, 0), 0] == b)
+        return bool(np.all(ok_prev & ok_next))
 
     def cfl_dt(self, cfl):
         check(lib().m3p_euler_cfl_dt_blocks(_ptr(self.u), self.mesh.nblk, self.mesh.n, _ptr(self.dl), self.gamma,
@@ -450,8 +485,9 @@ def cfl_step_pair_fused(solver, cfl):
 
 def cfl_step_pair_rk2(solver, cfl):
     """cfl_step_pair_fused with each step one fused RK2 launch (an
-    Euler2DSolver in mixed precision): the launch also reduces the next
-    step's CFL time step into solver.dt_dev.  Same results as cfl_step_pair."""
+    Euler2DSolver or Euler1DEnsembleSolver in mixed precision): the launch
+    also reduces the next step's CFL time step into solver.dt_dev.  Same
+    results as cfl_step_pair."""
     u, u2 = solver.u, solver.u2
     dt = solver.dt_dev
     for a, b in ((u, u2), (u2, u)):
