@@ -198,5 +198,8 @@ def check_kernel_errors(stream=0):
     check(lib().m3p_check_errors(_P(stream), ctypes.byref(code), ctypes.byref(block), ctypes.byref(zone)))
     if code.value == M3P_ERR_NEGATIVE_DENSITY:
         raise NegativeDensityError(code.value, "euler::recover_primitive (negative density) in block %d" % block.value)
+    if code.value == M3P_ERR_BAD_SHAPE:
+        raise M3PError(code.value, "block %d: its prev/next links are not mutual (the fused 1D RK2 step needs them "
+                                   "to be)" % block.value)
     if code.value:
         raise M3PError(code.value, "kernel error %d in block %d" % (code.value, block.value))
