@@ -708,6 +708,86 @@ mhd_face_b_kernel(MhdArgs a, FaceE<R> fe)
     if (k == N - 1) mhd_update_face<R, 3>(a, fe, dt, i, j, N);
 }
 
+// The face-field update by bricks, for cubes the column-tile pass covers: a
+// workgroup owns a BI x BJ x BK brick of cells and takes each edge electric
+// field it needs once (edge_emf_from_face, :76-98, into LDS), where
+// mhd_face_b_kernel takes it once for each of the four faces that share the
+// edge: the x edges of its BI cell planes ((BJ + 1) x (BK + 1) each), the y
+// and z edges of its BI + 1 x-face planes (BJ x (BK + 1), (BJ + 1) x BK).
+// Then each thread updates the three faces below its cell, and the faces at
+// N on the last planes, with the operands and arithmetic of
+// mhd_update_face.  A thread's face fields and RK base are loaded before the
+// edges, so their latency overlaps the edge loads.
+template<typename R, int BI, int BJ, int BK>
+__global__ void __launch_bounds__(BI * BJ * BK)
+mhd_face_b_brick_kernel(MhdArgs a, FaceE<R> fe)
+{
+    constexpr int NT = BI * BJ * BK;
+    constexpr int N1 = BI * (BJ + 1) * (BK + 1);        // x edges
+    constexpr int N2 = (BI + 1) * BJ * (BK + 1);        // y edges
+    constexpr int N3 = (BI + 1) * (BJ + 1) * BK;        // z edges
+    __shared__ double X1[BI][BJ + 1][BK + 1];           // ee1(i0 + p, j0 + r, k0 + c)
+    __shared__ double Y2[BI + 1][BJ][BK + 1];           // ee2(i0 + p, j0 + r, k0 + c)
+    __shared__ double Z3[BI + 1][BJ + 1][BK];           // ee3(i0 + p, j0 + r, k0 + c)
+    const int N = a.N;
+    int w = blockIdx.x;
+    const int k0 = (w % (N / BK)) * BK; w /= N / BK;
+    const int j0 = (w % (N / BJ)) * BJ; w /= N / BJ;
+    const int i0 = w * BI;
+    const int t = threadIdx.x;
+    const int il = t / (BJ * BK), jl = t / BK % BJ, kl = t % BK;
+    const int i = i0 + il, j = j0 + jl, k = k0 + kl;
+
+    const long fx = ((long)i * N + j) * N + k;
+    const long fy = ((long)i * (N + 1) + j) * N + k;
+    const long fz = ((long)i * N + j) * (N + 1) + k;
+    const double bx = a.b1[fx], by = a.b2[fy], bz = a.b3[fz];
+    double b0x = 0.0, b0y = 0.0, b0z = 0.0;
+    if (a.u0) { b0x = a.b01[fx]; b0y = a.b02[fy]; b0z = a.b03[fz]; }
+
+    for (int e = t; e < N1 + N2 + N3; e += NT) {
+        if (e < N1) {
+            const int c = e % (BK + 1), r = e / (BK + 1) % (BJ + 1), p = e / ((BK + 1) * (BJ + 1));
+            X1[p][r][c] = fe.ee1(i0 + p, j0 + r, k0 + c);
+        } else if (e < N1 + N2) {
+            const int f = e - N1;
+            const int c = f % (BK + 1), r = f / (BK + 1) % BJ, p = f / ((BK + 1) * BJ);
+            Y2[p][r][c] = fe.ee2(i0 + p, j0 + r, k0 + c);
+        } else {
+            const int f = e - N1 - N2;
+            const int c = f % BK, r = f / BK % (BJ + 1), p = f / (BK * (BJ + 1));
+            Z3[p][r][c] = fe.ee3(i0 + p, j0 + r, k0 + c);
+        }
+    }
+    __syncthreads();
+
+    const double dtdl = *a.dt / a.dl;
+    // mhd_update_face's output: B - curl(E) dt / dl, and the RK combine
+    auto put = [&] (double b, double b0, double* out, long f, double cf) {
+        double r = b - cf * dtdl;
+        if (a.u0) r = b0 * a.rk_b + r * (1.0 - a.rk_b);
+        out[f] = r;
+    };
+    put(bx, b0x, a.bo1, fx, (Z3[il][jl + 1][kl] - Z3[il][jl][kl]) - (Y2[il][jl][kl + 1] - Y2[il][jl][kl]));
+    put(by, b0y, a.bo2, fy, (X1[il][jl][kl + 1] - X1[il][jl][kl]) - (Z3[il + 1][jl][kl] - Z3[il][jl][kl]));
+    put(bz, b0z, a.bo3, fz, (Y2[il + 1][jl][kl] - Y2[il][jl][kl]) - (X1[il][jl + 1][kl] - X1[il][jl][kl]));
+    if (j == N - 1) {   // y face (i, N, k)
+        const long f = ((long)i * (N + 1) + N) * N + k;
+        put(a.b2[f], a.u0 ? a.b02[f] : 0.0, a.bo2, f,
+            (X1[il][BJ][kl + 1] - X1[il][BJ][kl]) - (Z3[il + 1][BJ][kl] - Z3[il][BJ][kl]));
+    }
+    if (k == N - 1) {   // z face (i, j, N)
+        const long f = ((long)i * N + j) * (N + 1) + N;
+        put(a.b3[f], a.u0 ? a.b03[f] : 0.0, a.bo3, f,
+            (Y2[il + 1][jl][BK] - Y2[il][jl][BK]) - (X1[il][jl + 1][BK] - X1[il][jl][BK]));
+    }
+    if (i == N - 1) {   // x face (N, j, k)
+        const long f = ((long)N * N + j) * N + k;
+        put(a.b1[f], a.u0 ? a.b01[f] : 0.0, a.bo1, f,
+            (Z3[BI][jl + 1][kl] - Z3[BI][jl][kl]) - (Y2[BI][jl][kl + 1] - Y2[BI][jl][kl]));
+    }
+}
+
 // max over cells of |div B| = |d1(B1) + d2(B2) + d3(B3)| (divergence_difference)
 static __global__ void __launch_bounds__(256)
 mhd_divb_kernel(const double* b1, const double* b2, const double* b3, int N, unsigned long long* slot)
