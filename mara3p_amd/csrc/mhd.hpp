@@ -261,7 +261,13 @@ mhd_prim_kernel(MhdArgs a, R* P)
     const long N3 = (long)N * N * N;
     const long NN = (long)N * N;
     double s = 0.0;
-    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < N3; c += (long)gridDim.x * blockDim.x) {
+    // FP64: each XCD sweeps a contiguous range of the grid's cells, so the
+    // face fields that neighbouring rows and planes share stay in its L2
+    // (step 0.9 % faster at 256^3).  Mixed keeps the hardware order, which
+    // writes the top planes last for the tile pass that starts there (the
+    // remap made its step 1.1 % slower).
+    const long w = is_fp64<R>::value ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    for (long c = w * blockDim.x + threadIdx.x; c < N3; c += (long)gridDim.x * blockDim.x) {
         const int i = (int)(c / NN), j = (int)(c / N % N), k = (int)(c % N);
         double b[3] = { 0.5 * (a.b1[(long)i * NN + (long)j * N + k] + a.b1[(long)(i + 1) * NN + (long)j * N + k]),
                         0.5 * (a.b2[((long)i * (N + 1) + j) * N + k] + a.b2[((long)i * (N + 1) + j + 1) * N + k]),
