@@ -524,9 +524,25 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
 #pragma unroll
                 for (int m = 0; m < NC; ++m) rowF[m][TJ] = hF[m][kl];
             }
+            // The update's loads (the zone's state where the kernel does not
+            // keep it in registers, the RK base where the loader has no
+            // pipeline) are issued before the barrier, which orders memory
+            // and so would hold them after it, so that their latency overlaps
+            // the wait (FP64 KH and Sedov steps 0.6 and 0.9 % faster).  A
+            // scheme with kLateBase keeps its RK base load at the update: held
+            // across the barrier it spills 20 registers.
+            double uu[NC];
+            if constexpr (!KEEPU) ld.cons_row(b, k, c, uu);
+            if constexpr (!Ld::kPrefetch && !S::kLateBase) {
+                if (!FIRST && a.u0) {
+                    const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) U0[m] = __builtin_nontemporal_load(&z0[(size_t)m * n * n]);
+                }
+            }
             __syncthreads();
 
-            double dfx[NC], dfy[NC], uu[NC], res[NC], fa[NC];
+            double dfx[NC], dfy[NC], res[NC], fa[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
                 fa[m] = double(Fa[m]);
@@ -536,8 +552,6 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             if constexpr (KEEPU) {
 #pragma unroll
                 for (int m = 0; m < NC; ++m) uu[m] = Ua[m];
-            } else {
-                ld.cons_row(b, k, c, uu);
             }
             S::update(prm, td, tc, k, c, uu, Pa, dfx, dfy, dt, dl, res);
             // The RK base and the output are touched once per launch:
@@ -546,7 +560,7 @@ stage2d_kernel(typename S::Params prm, Ld ld_arg, StageArgs a)
             double* o = a.out + ((size_t)b * NC * n + k) * n + c;
             double wv[NC];
             if (!FIRST && a.u0) {
-                if constexpr (!Ld::kPrefetch || S::kLateBase) {
+                if constexpr (S::kLateBase) {
                     const double* z0 = a.u0 + ((size_t)b * NC * n + k) * n + c;
 #pragma unroll
                     for (int m = 0; m < NC; ++m) U0[m] = __builtin_nontemporal_load(&z0[(size_t)m * n * n]);
