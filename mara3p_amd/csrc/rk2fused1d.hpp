@@ -48,6 +48,11 @@ rk2_fused_1d_kernel(EulerParams prm, Stage1DArgs a)
         return;
     }
     const double dl = a.dl_blk ? a.dl_blk[b] : a.dl;
+    // the cell spacing of the neighbours whose edge zones this tile updates
+    // in the first stage, loaded here rather than after a barrier, where
+    // its latency held up the workgroup (launch 2 % faster)
+    const double dl_lo = lo_edge && pb >= 0 && a.dl_blk ? a.dl_blk[pb] : dl;
+    const double dl_hi = hi_edge && nb >= 0 && a.dl_blk ? a.dl_blk[nb] : dl;
 
     __shared__ R P0[NC][TJ + 8];   // [i] = zone c0 - 4 + i: U0 primitives
     __shared__ R F1[NC][TJ + 5];   // [i] = face c0 - 2 + i (the left face of zone c0 - 2 + i): first stage
@@ -136,7 +141,7 @@ rk2_fused_1d_kernel(EulerParams prm, Stage1DArgs a)
         const bool hi_x = (t == 4 || t == 5) && !hi_out;
         if (lo_x || hi_x) {
             const int j = lo_x ? t - 2 : TJ + (t - 2);
-            const double dlz = a.dl_blk ? a.dl_blk[lo_x ? (lo_edge ? pb : b) : (hi_edge ? nb : b)] : a.dl;
+            const double dlz = lo_x ? dl_lo : dl_hi;
             double g1[NC];
             update1(j, gu0, dlz, g1);
             ok &= Euler<R>::recover(g1, prm, p);
