@@ -403,9 +403,12 @@ mhd_boundary_faces_kernel(MhdArgs a, const R* P, R* F1, R* E1, R* F2, R* E2, R* 
 // of one segment.  Each thread keeps a four-plane window of its column's
 // primitives, so the x face above its cell needs one new load per plane;
 // the y and z faces of the plane come from an LDS copy of the plane with a
-// two-cell halo.  Per plane and thread: the x face above the cell, the y
-// and z faces below it (the tile's upper y and z faces by the last row and
-// by the first TJ threads), then U of the cell from the six face fluxes
+// two-cell halo.  Per plane and thread: the y and z faces below the cell
+// (the tile's upper y and z faces by the last row and by the first TJ
+// threads), the x face above it (after the y and z faces, so the next
+// plane's primitives it needs arrive meanwhile; FP64 then holds 222 VGPRs
+// instead of 256, and the RK2 step at 256^3 is 0.5 % (mixed) and 0.9 %
+// (FP64) faster), then U of the cell from the six face fluxes
 // (divergence_difference, src/mesh_cartesian_3d.hpp:290-301).  Face fluxes
 // never leave the workgroup; the transverse face electric fields go to the
 // E arrays for the face update.  Every face is computed from the same
@@ -528,23 +531,6 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
                 for (int q = 0; q < 8; ++q) Pl[q][rj][rk] = P[q * N3 + c];
             }
         }
-
-        // x face above the cell (face i + 1)
-        R Fx_hi[5];
-        {
-            R pl[8], pr[8], e[3];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const R gp = mhd_half_slope(P0[q], Pp[q], Pn[q], theta);
-                pl[q] = P0[q] + g0[q];
-                pr[q] = Pp[q] - gp;
-                g0[q] = gp;
-            }
-            const long f = ((long)(i + 1) * N + j) * N + k;
-            Mhd<R>::template riemann<1>(pl, pr, R(a.b1[f]), gamma, Fx_hi, e);
-            E1[f] = e[1];
-            E1[nf + f] = e[2];
-        }
         __syncthreads();
 
         // y face below the cell (face j), and the tile's upper y face by the last row
@@ -582,6 +568,22 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
         };
         zface(jl, kl);
         if (t < TJ) zface(t, TK);
+        // x face above the cell (face i + 1)
+        R Fx_hi[5];
+        {
+            R pl[8], pr[8], e[3];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const R gp = mhd_half_slope(P0[q], Pp[q], Pn[q], theta);
+                pl[q] = P0[q] + g0[q];
+                pr[q] = Pp[q] - gp;
+                g0[q] = gp;
+            }
+            const long f = ((long)(i + 1) * N + j) * N + k;
+            Mhd<R>::template riemann<1>(pl, pr, R(a.b1[f]), gamma, Fx_hi, e);
+            E1[f] = e[1];
+            E1[nf + f] = e[2];
+        }
         // the update's U and RK base, loaded before the barrier (which
         // orders memory, so the compiler keeps loads written after it there)
         // so that their latency overlaps the wait: the 256^3 RK2 step 5.7 %
