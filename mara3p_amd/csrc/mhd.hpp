@@ -536,6 +536,9 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
         // step 1.3 % faster at 256^3); the mixed pass loads them in the face
         // (0.2 % faster there)
         constexpr bool kPreB = is_fp64<R>::value;
+        // the x face's longitudinal field (its face follows the barrier), in
+        // both precisions (FP64 step 1.0 % faster, mixed unchanged)
+        const R blx = R(a.b1[((long)(i + 1) * N + j) * N + k]);
         R bly = R(0), blz = R(0);
         if constexpr (kPreB) {
             bly = R(a.b2[((long)i * (N + 1) + j) * N + k]);
@@ -590,7 +593,7 @@ mhd_tile_kernel(MhdArgs a, const R* P, R* E1, R* E2, R* E3, int TI)
                 g0[q] = gp;
             }
             const long f = ((long)(i + 1) * N + j) * N + k;
-            Mhd<R>::template riemann<1>(pl, pr, R(a.b1[f]), gamma, Fx_hi, e);
+            Mhd<R>::template riemann<1>(pl, pr, blx, gamma, Fx_hi, e);
             E1[f] = e[1];
             E1[nf + f] = e[2];
         }
