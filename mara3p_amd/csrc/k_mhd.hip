@@ -41,7 +41,7 @@ int mhd_advance(MhdArgs a, char* work, hipStream_t s)
     FaceE<R> fe{ E1, E2, E3, N, nf };
     if (tiled) {
         // column tiles: fluxes and the conserved update in one pass, then the
-        // faces.  FP64 too (256 VGPRs at two waves per SIMD, no spills): at
+        // faces.  FP64 too (236 VGPRs at two waves per SIMD, no spills): at
         // 256^3 its RK2 step takes 5.77 ms against 7.06 ms through the
         // per-cell passes below, which write every face flux and read it back.
         // 16-plane segments: 4096 workgroups at 256^3, four full rounds of

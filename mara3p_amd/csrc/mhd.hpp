@@ -406,9 +406,9 @@ mhd_boundary_faces_kernel(MhdArgs a, const R* P, R* F1, R* E1, R* F2, R* E2, R* 
 // two-cell halo.  Per plane and thread: the y and z faces below the cell
 // (the tile's upper y and z faces by the last row and by the first TJ
 // threads), the x face above it (after the y and z faces, so the next
-// plane's primitives it needs arrive meanwhile; FP64 then holds 222 VGPRs
-// instead of 256, and the RK2 step at 256^3 is 0.5 % (mixed) and 0.9 %
-// (FP64) faster), then U of the cell from the six face fluxes
+// plane's primitives it needs arrive meanwhile; FP64 needs 34 fewer
+// VGPRs, and the RK2 step at 256^3 is 0.5 % (mixed) and 0.9 % (FP64)
+// faster), then U of the cell from the six face fluxes
 // (divergence_difference, src/mesh_cartesian_3d.hpp:290-301).  Face fluxes
 // never leave the workgroup; the transverse face electric fields go to the
 // E arrays for the face update.  Every face is computed from the same
