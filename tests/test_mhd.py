@@ -115,7 +115,9 @@ def _rel(a, b):
 def test_gpu_mhd_advance_ct_fp64_matches_oracle(N):
     """Two FP64 RK2 steps against the oracle: N = 12 runs the per-cell passes,
     8 and 16 the 8 x 8 column tiles, 128 the 8 x 32 tiles in 16-plane
-    segments, 72 (N % 16 == 8) the 8 x 8 tiles in 8-plane segments."""
+    segments, 72 (N % 16 == 8) the 8 x 8 tiles in 8-plane segments; the
+    tiled sizes update the face fields by bricks (2 x 4 x 8 cells, and
+    4 x 4 x 32 at N = 128)."""
     from mara3p_amd.problems import orszag_tang_3d
     from mara3p_amd.solvers import MhdSolver
     uc, b1, b2, b3 = orszag_tang_3d(N, seed=2, perturbation=0.05)
